@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: SGEMM 8192 x 8192 x 8192, BF16x3 split precision vs plain FP32
+MFMA, with normwise error against an FP64 reference.
+
+One step = one C = A*B over the resident 8192^3 operands (A, B uniform[-1,1]
+from MT19937 seed 1, beta = 0).  `value` is the BF16x3 GEMM's throughput
+(2*M*N*K useful flops per step); the plain FP32-MFMA GEMM is timed the same
+way in the same run and reported beside it, with both normwise errors
+||C - C64||_F / (||A||_F ||B||_F) computed in FP64 on the GPU against an FP64
+MFMA product of the same inputs (itself spot-checked against numpy float64
+rows).  The cpu_baseline leg times the reference's CPU verification loop
+(Matrix<T>::operator*, Kernel/cuda/src/Matrix.h:183-201, restated in the
+oracle) on a bounded row sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SGEMM TFLOP/s + % peak: BF16x3-MFMA vs plain FP32, at normwise err vs FP64 oracle"
+PEAK_BF16 = 2516.6  # TFLOP/s, MI355X dense bf16 MFMA
+PEAK_BF16X3 = PEAK_BF16 / 3.0  # useful-flop peak of three bf16 products per FP32 product
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 MFMA
+M = N = K = 8192
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_launches(fn, stream, steps: int, warmup: int, torch):
+    """Warm up, then time `steps` launches with one HIP event pair per launch
+    on the launch stream; return (wall ms per step, mean event ms per launch)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        fn()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    per = [s.elapsed_time(e) for s, e in ev]
+    return wall, float(np.mean(per)), float(np.min(per))
+
+
+def cpu_baseline(A: np.ndarray, B: np.ndarray, target_s: float = 12.0) -> dict:
+    """Time the reference's CPU verification product on a row sample."""
+    import oracle  # the CPU restatement; used here only as the timed baseline
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rows = 2
+    while True:
+        t0 = time.perf_counter()
+        oracle.cpu_naive_sgemm(A, B, rows=rows, row0=0, nthreads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= target_s / 4 or rows >= 1024:
+            break
+        rows = min(1024, max(rows * 2, int(rows * target_s / 4 / max(dt, 1e-3))))
+    flops = 2.0 * rows * N * K
+    t0 = time.perf_counter()
+    oracle.cpu_naive_sgemm(A, B, rows=1, row0=rows, nthreads=1)
+    dt1 = time.perf_counter() - t0
+    return {
+        "value": flops / dt / 1e12,
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"rows 0..{rows - 1} of the {M}x{N}x{K} product (naive i-j-k, float "
+                   f"accumulator, OpenMP over rows, {threads} threads) in {dt:.1f} s; "
+                   f"1 row on 1 core: {2.0 * N * K / dt1 / 1e9:.3f} GFLOP/s; extrapolated"),
+        "single_core_value": 2.0 * N * K / dt1 / 1e12,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 error reports")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import cucosma_amd as cc
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("generating inputs (numpy RandomState(1), uniform[-1,1])")
+    rs = np.random.RandomState(1)
+    A = rs.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rs.uniform(-1, 1, (K, N)).astype(np.float32)
+    Ad = torch.from_numpy(A).to(dev)
+    Bd = torch.from_numpy(B).to(dev)
+    Cb = torch.empty((M, N), dtype=torch.float32, device=dev)
+    Cp = torch.empty((M, N), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    lib = cc.lib
+    a, b, cb, cp = Ad.data_ptr(), Bd.data_ptr(), Cb.data_ptr(), Cp.data_ptr()
+
+    def run_bf16x3():
+        st = lib.cosma_sgemm_bf16x3_8192x8192x8192(1.0, a, K, b, N, 0.0, cb, N)
+        if st:
+            raise cc.CosmaError(st, "bf16x3")
+
+    def run_fp32():
+        st = lib.cosma_sgemm_8192x8192x8192(1.0, a, K, b, N, 0.0, cp, N)
+        if st:
+            raise cc.CosmaError(st, "fp32")
+
+    # Both fixed-shape entry points launch on the default (null) stream, which
+    # is torch's current stream here.
+    flops = 2.0 * M * N * K
+    log(f"timing bf16x3: warmup {args.warmup}, steps {args.steps}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    wall_b, ev_b, min_b = time_launches(run_bf16x3, stream, args.steps, args.warmup, torch)
+    torch.cuda.synchronize()
+    log(f"bf16x3: {ev_b:.3f} ms/launch ({flops / ev_b / 1e9:.1f} TFLOP/s) in {time.perf_counter() - t0:.1f} s")
+    wall_p, ev_p, min_p = time_launches(run_fp32, stream, args.steps, args.warmup, torch)
+    log(f"fp32:   {ev_p:.3f} ms/launch ({flops / ev_p / 1e9:.1f} TFLOP/s)")
+
+    tf_b = flops / wall_b / 1e9
+    tf_p = flops / wall_p / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(tf_b, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_b, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3 (fp32 operands split to bf16 hi/lo, fp32 accumulate)",
+        "data": "synthetic: A, B uniform[-1,1] from MT19937 seed 1 (numpy RandomState), beta=0",
+        "config": {"workload": "sgemm 8192x8192x8192 row-major, alpha=1 beta=0, split-precision "
+                               "bf16x3 vs plain fp32 mfma", "M": M, "N": N, "K": K},
+        "bf16x3": {"tflops": round(tf_b, 3), "pct_useful_peak": round(100 * tf_b / PEAK_BF16X3, 2),
+                   "pct_bf16_mfma_peak": round(100 * 3 * tf_b / PEAK_BF16, 2),
+                   "ms_per_launch_events": round(ev_b, 4)},
+        "plain_fp32": {"tflops": round(tf_p, 3), "pct_peak": round(100 * tf_p / PEAK_FP32, 2),
+                       "ms_per_launch_events": round(ev_p, 4)},
+        "speedup_bf16x3_vs_fp32": round(tf_b / tf_p, 3),
+        "roofline": {"bound": "mfma", "achieved": round(flops / ev_b / 1e9, 3),
+                     "peak": round(PEAK_BF16X3, 1), "unit": "TFLOP/s",
+                     "frac": round(flops / ev_b / 1e9 / PEAK_BF16X3, 4), "traffic": None,
+                     "kernel": "gemm_kernel<BF16x3, 128x128x32>",
+                     "peak_basis": "MI355X dense bf16 MFMA 2516.6 TF / 3 products"},
+    }
+
+    if not args.no_check:
+        log("FP64 reference on the GPU (FP64 MFMA on exact copies)")
+        R = cc.fp64_product(Ad, Bd)
+        torch.cuda.synchronize()
+        rows = [0, 4097, 8191]
+        ref_rows = A[rows].astype(np.float64) @ B.astype(np.float64)
+        dev_rows = R[rows].cpu().numpy()
+        spot = float(np.max(np.abs(dev_rows - ref_rows)) / np.max(np.abs(ref_rows)))
+        eb = cc.normwise_error(Ad, Bd, Cb, R)
+        ep = cc.normwise_error(Ad, Bd, Cp, R)
+        result["bf16x3"]["normwise_err"] = eb["normwise"]
+        result["bf16x3"]["max_rel_err"] = eb["max_rel"]
+        result["plain_fp32"]["normwise_err"] = ep["normwise"]
+        result["plain_fp32"]["max_rel_err"] = ep["max_rel"]
+        result["err_ratio_bf16x3_vs_fp32"] = eb["normwise"] / ep["normwise"] if ep["normwise"] else None
+        result["fp64_reference_spotcheck_rel"] = spot
+        del R
+        log(f"normwise: bf16x3 {eb['normwise']:.3e}  fp32 {ep['normwise']:.3e}")
+
+    if not args.no_cpu:
+        log("cpu baseline (reference CPU verification loop on a row sample)")
+        result["cpu_baseline"] = cpu_baseline(A, B)
+    else:
+        result["cpu_baseline"] = None
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

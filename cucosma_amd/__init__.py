@@ -1,0 +1,31 @@
+"""cucosma_amd -- MI355X (gfx950) COSMA-tiled GEMM: FP32 / FP64 / BF16x3 on MFMA.
+
+The compute lives in libcosma_amd.so (hand-written HIP kernels behind the C
+ABI of include/cosma/cosma.h).  Importing this package loads that library and
+raises ImportError when it has not been built; nothing falls back to another
+implementation.
+"""
+from ._lib import (  # noqa: F401
+    MODE_BF16X3,
+    MODE_FP32,
+    MODE_FP64,
+    CosmaError,
+    check,
+    heuristic,
+    lib,
+    tile_configs,
+)
+from .gemm import (  # noqa: F401
+    dgemm,
+    error_stats,
+    gemm,
+    normwise_error,
+    fp64_product,
+    sgemm,
+    sgemm_bf16x3,
+    sumsq,
+    to_f64,
+    workspace_bytes,
+)
+
+__version__ = lib.cosma_version().decode()

@@ -1,0 +1,131 @@
+"""ctypes binding of libcosma_amd.so (the C ABI in include/cosma/cosma.h).
+
+The library is built in-tree by ``python -m cucosma_amd.build``.  Importing
+this module fails loudly when it is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcosma_amd.so")
+
+STATUS_SUCCESS = 0
+STATUS_INVALID_VALUE = -1
+STATUS_MISALIGNED = -2
+STATUS_NOT_SUPPORTED = -3
+STATUS_WORKSPACE = -4
+
+MODE_FP32 = 0
+MODE_FP64 = 1
+MODE_BF16X3 = 2
+MODES = {"fp32": MODE_FP32, "fp64": MODE_FP64, "bf16x3": MODE_BF16X3}
+
+
+class CosmaError(RuntimeError):
+    """A non-zero status returned by the C ABI."""
+
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        text = _lib.cosma_status_string(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: status {status} ({text})" if what else f"status {status} ({text})")
+
+
+class GemmDesc(C.Structure):
+    """Mirror of struct cosma_gemm_desc."""
+
+    _fields_ = [
+        ("mode", C.c_int),
+        ("m", C.c_int),
+        ("n", C.c_int),
+        ("k", C.c_int),
+        ("trans_a", C.c_int),
+        ("trans_b", C.c_int),
+        ("a", C.c_void_p),
+        ("lda", C.c_int64),
+        ("b", C.c_void_p),
+        ("ldb", C.c_int64),
+        ("c", C.c_void_p),
+        ("ldc", C.c_int64),
+        ("alpha", C.c_double),
+        ("beta", C.c_double),
+        ("split_k", C.c_int),
+        ("tile_cfg", C.c_int),
+        ("group_m", C.c_int),
+        ("workspace", C.c_void_p),
+        ("workspace_bytes", C.c_size_t),
+        ("stream", C.c_void_p),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first (python -m cucosma_amd.build)"
+        )
+    lib = C.CDLL(LIB_PATH)
+    f32p, f64p, vp = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_void_p
+    i, i64, d, f = C.c_int, C.c_int64, C.c_double, C.c_float
+    sig = {
+        "cosma_gemm_desc_init": (None, [C.POINTER(GemmDesc)]),
+        "cosma_gemm_ex": (i, [C.POINTER(GemmDesc)]),
+        "cosma_workspace_query": (i, [C.POINTER(GemmDesc), C.POINTER(C.c_size_t)]),
+        "cosma_workspace_set": (i, [vp, C.c_size_t]),
+        "cosma_sgemm": (i, [f, vp, i, vp, i, f, vp, i]),
+        "cosma_dgemm": (i, [d, vp, i, vp, i, d, vp, i]),
+        "cosma_fixed_shape": (None, [C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "cosma_sgemm_1024x1024x1024": (i, [f, vp, i, vp, i, f, vp, i]),
+        "cosma_sgemm_8192x8192x8192": (i, [f, vp, i, vp, i, f, vp, i]),
+        "cosma_sgemm_256x256x1048576": (i, [f, vp, i, vp, i, f, vp, i]),
+        "cosma_sgemm_bf16x3_8192x8192x8192": (i, [f, vp, i, vp, i, f, vp, i]),
+        "cosma_sgemm_bf16x3_256x256x1048576": (i, [f, vp, i, vp, i, f, vp, i]),
+        "cosma_dgemm_32768x1024x1024": (i, [d, vp, i, vp, i, d, vp, i]),
+        "cosma_dgemm_1024x65536x1024": (i, [d, vp, i, vp, i, d, vp, i]),
+        "cosma_sgemm_mnk": (i, [i, i, i, f, vp, i, vp, i, f, vp, i, vp]),
+        "cosma_dgemm_mnk": (i, [i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
+        "cosma_sgemm_bf16x3_mnk": (i, [i, i, i, f, vp, i, vp, i, f, vp, i, vp]),
+        "cosma_num_tile_configs": (i, [i]),
+        "cosma_tile_config": (i, [i, i, C.POINTER(i)]),
+        "cosma_heuristic": (i, [i, i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "cosma_error_stats_f32": (i, [i, i, vp, i64, vp, i64, f64p, vp]),
+        "cosma_error_stats_f64": (i, [i, i, vp, i64, vp, i64, f64p, vp]),
+        "cosma_sumsq_f32": (i, [i, i, vp, i64, f64p, vp]),
+        "cosma_sumsq_f64": (i, [i, i, vp, i64, f64p, vp]),
+        "cosma_convert_f32_f64": (i, [i, i, vp, i64, vp, i64, vp]),
+        "cosma_status_string": (C.c_char_p, [i]),
+        "cosma_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    del f32p
+    return lib
+
+
+_lib = None
+_lib = _load()
+lib = _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != STATUS_SUCCESS:
+        raise CosmaError(status, what)
+
+
+def tile_configs(mode: int) -> list[dict]:
+    """Tile configurations built into the library for a mode."""
+    out = []
+    arr = (C.c_int * 7)()
+    for cfg in range(lib.cosma_num_tile_configs(mode)):
+        check(lib.cosma_tile_config(mode, cfg, arr), "cosma_tile_config")
+        bm, bn, bk, wm, wn, thr, lds = list(arr)
+        out.append(dict(cfg=cfg, BM=bm, BN=bn, BK=bk, WM=wm, WN=wn, threads=thr, lds_bytes=lds))
+    return out
+
+
+def heuristic(mode: int, m: int, n: int, k: int) -> tuple[int, int]:
+    cfg, split = C.c_int(), C.c_int()
+    check(lib.cosma_heuristic(mode, m, n, k, C.byref(cfg), C.byref(split)), "cosma_heuristic")
+    return cfg.value, split.value

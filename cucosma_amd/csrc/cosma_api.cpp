@@ -1,0 +1,477 @@
+// cosma_api.cpp -- the C ABI declared in include/cosma/cosma.h.
+//
+// Host side of the reference's cosmaSgemm (Kernel/cuda/src/cuCOSMAV100.cuh:2774-2848):
+// argument checks, the alpha == 0 / beta pre-scale branch (:2782-2797), tile
+// and split-K choice (:2797-2822, one build per config.h row in the
+// reference), split-K workspace (:2824-2838) and the launches.  Every check
+// that can fail runs before the first HIP call, so a contract violation
+// returns a negative status without touching the GPU.
+#include <hip/hip_runtime_api.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/cosma/cosma.h"
+#include "cosma_internal.hpp"
+
+#ifndef COSMA_M
+#define COSMA_M 1024
+#endif
+#ifndef COSMA_N
+#define COSMA_N 1024
+#endif
+#ifndef COSMA_K
+#define COSMA_K 1024
+#endif
+
+namespace {
+
+using cosma::Problem;
+
+constexpr int kMaxSplit = 4096;
+constexpr int kCUs = 256;  // MI355X compute units
+
+// Tile extents of the built configurations, mirrored as constants so the
+// fixed-shape entry points can pick them at compile time.  Must match the
+// tables in cosma_f32.hip / cosma_f64.hip / cosma_bf16x3.hip (checked at
+// first use by check_tables()).
+struct TileDims {
+  int bm, bn, bk;
+};
+constexpr TileDims kDims[3][4] = {
+    {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}},  // FP32
+    {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {0, 0, 0}},       // FP64
+    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {0, 0, 0}},      // BF16X3
+};
+constexpr int kNumCfg[3] = {4, 3, 3};
+
+constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+constexpr int tiles_of(int mode, int cfg, int m, int n) {
+  return cdiv(m, kDims[mode][cfg].bm) * cdiv(n, kDims[mode][cfg].bn);
+}
+
+// Heuristic tile / split choice (replaced per shape by the autotuner's
+// measured choice when the caller passes tile_cfg / split_k).
+// Large outputs take the 128x128 tile; outputs too small to give every CU
+// work take smaller tiles and then split K so that about two workgroups per
+// CU exist, keeping every K slice at least 512 deep.
+struct Choice {
+  int cfg, split;
+};
+constexpr Choice heuristic(int mode, int m, int n, int k) {
+  int cfg = 0;
+  if (tiles_of(mode, 0, m, n) < 2 * kCUs) {
+    cfg = (mode == 2) ? 1 : 2;  // 64x64 (bf16x3) or 128x64
+    if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;
+  }
+  int tiles = tiles_of(mode, cfg, m, n);
+  int split = 1;
+  if (tiles < kCUs && k >= 1024) {
+    split = cdiv(2 * kCUs, tiles);
+    int max_by_depth = k / 512;
+    if (split > max_by_depth) split = max_by_depth;
+    if (split > 64) split = 64;
+    if (split < 1) split = 1;
+  }
+  return Choice{cfg, split};
+}
+
+size_t elem_size(int mode) { return mode == COSMA_MODE_FP64 ? 8 : 4; }
+int vec_elems(int mode) { return mode == COSMA_MODE_FP64 ? 2 : 4; }
+
+std::mutex g_mu;
+struct DevState {
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  double* stat_part = nullptr;  // kStatParts * 5 doubles + 5 doubles of output
+};
+std::vector<DevState> g_dev;
+void* g_user_ws = nullptr;
+size_t g_user_ws_bytes = 0;
+
+DevState* dev_state(int* status) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *status = (int)e;
+    return nullptr;
+  }
+  if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+  return &g_dev[dev];
+}
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Resolve the split-K buffer: the descriptor's, the installed one, or the
+// library-owned one (allocated or grown only when the stream is not being
+// captured).
+int resolve_workspace(const cosma_gemm_desc* d, size_t need, hipStream_t s, void** out) {
+  if (d->workspace) {
+    if (d->workspace_bytes < need) return COSMA_STATUS_INVALID_VALUE;
+    *out = d->workspace;
+    return 0;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_user_ws && g_user_ws_bytes >= need) {
+    *out = g_user_ws;
+    return 0;
+  }
+  int status = 0;
+  DevState* ds = dev_state(&status);
+  if (!ds) return status;
+  if (ds->ws_bytes < need) {
+    if (capturing(s)) return COSMA_STATUS_WORKSPACE;
+    if (ds->ws) {
+      hipError_t e = hipDeviceSynchronize();
+      if (e != hipSuccess) return (int)e;
+      e = hipFree(ds->ws);
+      ds->ws = nullptr;
+      ds->ws_bytes = 0;
+      if (e != hipSuccess) return (int)e;
+    }
+    hipError_t e = hipMalloc(&ds->ws, need);
+    if (e != hipSuccess) {
+      ds->ws = nullptr;
+      return (int)e;
+    }
+    ds->ws_bytes = need;
+  }
+  *out = ds->ws;
+  return 0;
+}
+
+// Host-side contract check; returns 0, or a negative status.
+int validate(const cosma_gemm_desc* d) {
+  if (!d) return COSMA_STATUS_INVALID_VALUE;
+  if (d->mode < 0 || d->mode > 2) return COSMA_STATUS_INVALID_VALUE;
+  if (d->m < 0 || d->n < 0 || d->k < 0) return COSMA_STATUS_INVALID_VALUE;
+  if ((d->trans_a != 0 && d->trans_a != 1) || (d->trans_b != 0 && d->trans_b != 1))
+    return COSMA_STATUS_INVALID_VALUE;
+  if (d->split_k > kMaxSplit) return COSMA_STATUS_INVALID_VALUE;
+  if (d->tile_cfg >= kNumCfg[d->mode]) return COSMA_STATUS_INVALID_VALUE;
+  if (d->m == 0 || d->n == 0) return 0;
+  if (!d->c) return COSMA_STATUS_INVALID_VALUE;
+  if (d->ldc < (d->n > 1 ? d->n : 1)) return COSMA_STATUS_INVALID_VALUE;
+  const bool uses_ab = d->k > 0 && d->alpha != 0.0;
+  if (!uses_ab) return 0;
+  if (d->trans_a && d->trans_b) return COSMA_STATUS_NOT_SUPPORTED;
+  if (!d->a || !d->b) return COSMA_STATUS_INVALID_VALUE;
+  const int64_t a_cols = d->trans_a ? d->m : d->k;
+  const int64_t b_cols = d->trans_b ? d->k : d->n;
+  if (d->lda < (a_cols > 1 ? a_cols : 1) || d->ldb < (b_cols > 1 ? b_cols : 1))
+    return COSMA_STATUS_INVALID_VALUE;
+  const int v = vec_elems(d->mode);
+  if (((uintptr_t)d->a & 15u) || ((uintptr_t)d->b & 15u) || (d->lda % v) || (d->ldb % v))
+    return COSMA_STATUS_MISALIGNED;
+  return 0;
+}
+
+int run(const cosma_gemm_desc* d) {
+  int st = validate(d);
+  if (st != 0) return st;
+  if (d->m == 0 || d->n == 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(d->stream);
+  const bool uses_ab = d->k > 0 && d->alpha != 0.0;
+  if (!uses_ab) {  // C = beta*C (reference: pre-scale only, cuCOSMAV100.cuh:2782-2797)
+    if (d->beta == 1.0) return 0;
+    hipError_t e = d->mode == COSMA_MODE_FP64
+                       ? cosma::run_scale_f64(static_cast<double*>(d->c), d->ldc, d->m, d->n, d->beta, s)
+                       : cosma::run_scale_f32(static_cast<float*>(d->c), d->ldc, d->m, d->n,
+                                              (float)d->beta, s);
+    return (int)e;
+  }
+  Choice ch = heuristic(d->mode, d->m, d->n, d->k);
+  Problem p;
+  p.mode = d->mode;
+  p.M = d->m;
+  p.N = d->n;
+  p.K = d->k;
+  p.ta = d->trans_a;
+  p.tb = d->trans_b;
+  p.A = d->a;
+  p.lda = d->lda;
+  p.B = d->b;
+  p.ldb = d->ldb;
+  p.C = d->c;
+  p.ldc = d->ldc;
+  p.alpha = d->alpha;
+  p.beta = d->beta;
+  p.cfg = d->tile_cfg >= 0 ? d->tile_cfg : ch.cfg;
+  p.split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
+  if (p.split > d->k) p.split = d->k > 0 ? d->k : 1;
+  p.group_m = d->group_m;
+  p.ws = nullptr;
+  p.stream = s;
+  if (p.split > 1) {
+    const size_t need = (size_t)p.split * (size_t)d->m * (size_t)d->n * elem_size(d->mode);
+    st = resolve_workspace(d, need, s, &p.ws);
+    if (st != 0) return st;
+  }
+  hipError_t e;
+  switch (d->mode) {
+    case COSMA_MODE_FP32: e = cosma::run_gemm_f32(p); break;
+    case COSMA_MODE_FP64: e = cosma::run_gemm_f64(p); break;
+    default: e = cosma::run_gemm_bf16x3(p); break;
+  }
+  return (int)e;
+}
+
+cosma_gemm_desc make_desc(int mode, int m, int n, int k, double alpha, const void* A, int lda,
+                          const void* B, int ldb, double beta, void* C, int ldc, void* stream) {
+  cosma_gemm_desc d;
+  cosma_gemm_desc_init(&d);
+  d.mode = mode;
+  d.m = m;
+  d.n = n;
+  d.k = k;
+  d.a = A;
+  d.lda = lda;
+  d.b = B;
+  d.ldb = ldb;
+  d.c = C;
+  d.ldc = ldc;
+  d.alpha = alpha;
+  d.beta = beta;
+  d.stream = stream;
+  return d;
+}
+
+// Fixed-shape call: shape, tile configuration and split factor are
+// compile-time constants of the instantiation.
+template <int MODE, int M_, int N_, int K_>
+int fixed_gemm(double alpha, const void* A, int lda, const void* B, int ldb, double beta, void* C,
+               int ldc) {
+  static_assert(M_ > 0 && N_ > 0 && K_ > 0, "fixed shapes are positive");
+  constexpr Choice ch = heuristic(MODE, M_, N_, K_);
+  static_assert(ch.cfg >= 0 && ch.split >= 1, "heuristic");
+  cosma_gemm_desc d = make_desc(MODE, M_, N_, K_, alpha, A, lda, B, ldb, beta, C, ldc, nullptr);
+  d.tile_cfg = ch.cfg;
+  d.split_k = ch.split;
+  return run(&d);
+}
+
+int stats_scratch(double** part, double** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int status = 0;
+  DevState* ds = dev_state(&status);
+  if (!ds) return status;
+  if (!ds->stat_part) {
+    hipError_t e = hipMalloc(&ds->stat_part, sizeof(double) * (cosma::kStatParts * 5 + 5));
+    if (e != hipSuccess) {
+      ds->stat_part = nullptr;
+      return (int)e;
+    }
+  }
+  *part = ds->stat_part;
+  *out = ds->stat_part + cosma::kStatParts * 5;
+  return 0;
+}
+
+int finish_stats(hipError_t e, const double* dev_out, double* out5, hipStream_t s) {
+  if (e != hipSuccess) return (int)e;
+  e = hipMemcpyAsync(out5, dev_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return (int)e;
+  return (int)hipStreamSynchronize(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void cosma_gemm_desc_init(cosma_gemm_desc* d) {
+  if (!d) return;
+  d->mode = COSMA_MODE_FP32;
+  d->m = d->n = d->k = 0;
+  d->trans_a = d->trans_b = 0;
+  d->a = d->b = nullptr;
+  d->c = nullptr;
+  d->lda = d->ldb = d->ldc = 0;
+  d->alpha = 1.0;
+  d->beta = 0.0;
+  d->split_k = 0;
+  d->tile_cfg = -1;
+  d->group_m = 0;
+  d->workspace = nullptr;
+  d->workspace_bytes = 0;
+  d->stream = nullptr;
+}
+
+int cosma_gemm_ex(const cosma_gemm_desc* d) { return run(d); }
+
+int cosma_workspace_query(const cosma_gemm_desc* d, size_t* bytes) {
+  int st = validate(d);
+  if (st != 0) return st;
+  if (!bytes) return COSMA_STATUS_INVALID_VALUE;
+  *bytes = 0;
+  if (d->m == 0 || d->n == 0 || d->k == 0 || d->alpha == 0.0) return 0;
+  Choice ch = heuristic(d->mode, d->m, d->n, d->k);
+  int split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
+  if (split > d->k) split = d->k;
+  if (split > 1) *bytes = (size_t)split * (size_t)d->m * (size_t)d->n * elem_size(d->mode);
+  return 0;
+}
+
+int cosma_workspace_set(void* ptr, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_user_ws = bytes ? ptr : nullptr;
+  g_user_ws_bytes = ptr ? bytes : 0;
+  return 0;
+}
+
+int cosma_sgemm(float alpha, const float* A, int lda, const float* B, int ldb, float beta,
+                float* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP32, COSMA_M, COSMA_N, COSMA_K>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_dgemm(double alpha, const double* A, int lda, const double* B, int ldb, double beta,
+                double* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP64, COSMA_M, COSMA_N, COSMA_K>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+void cosma_fixed_shape(int* m, int* n, int* k) {
+  if (m) *m = COSMA_M;
+  if (n) *n = COSMA_N;
+  if (k) *k = COSMA_K;
+}
+
+int cosma_sgemm_1024x1024x1024(float alpha, const float* A, int lda, const float* B, int ldb,
+                               float beta, float* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP32, 1024, 1024, 1024>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_sgemm_8192x8192x8192(float alpha, const float* A, int lda, const float* B, int ldb,
+                               float beta, float* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP32, 8192, 8192, 8192>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_sgemm_256x256x1048576(float alpha, const float* A, int lda, const float* B, int ldb,
+                                float beta, float* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP32, 256, 256, 1048576>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_sgemm_bf16x3_8192x8192x8192(float alpha, const float* A, int lda, const float* B,
+                                      int ldb, float beta, float* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_BF16X3, 8192, 8192, 8192>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_sgemm_bf16x3_256x256x1048576(float alpha, const float* A, int lda, const float* B,
+                                       int ldb, float beta, float* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_BF16X3, 256, 256, 1048576>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_dgemm_32768x1024x1024(double alpha, const double* A, int lda, const double* B, int ldb,
+                                double beta, double* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP64, 32768, 1024, 1024>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int cosma_dgemm_1024x65536x1024(double alpha, const double* A, int lda, const double* B, int ldb,
+                                double beta, double* C, int ldc) {
+  return fixed_gemm<COSMA_MODE_FP64, 1024, 65536, 1024>(alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int cosma_sgemm_mnk(int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                    int ldb, float beta, float* C, int ldc, void* stream) {
+  cosma_gemm_desc d = make_desc(COSMA_MODE_FP32, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+  return run(&d);
+}
+int cosma_dgemm_mnk(int m, int n, int k, double alpha, const double* A, int lda, const double* B,
+                    int ldb, double beta, double* C, int ldc, void* stream) {
+  cosma_gemm_desc d = make_desc(COSMA_MODE_FP64, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+  return run(&d);
+}
+int cosma_sgemm_bf16x3_mnk(int m, int n, int k, float alpha, const float* A, int lda,
+                           const float* B, int ldb, float beta, float* C, int ldc, void* stream) {
+  cosma_gemm_desc d =
+      make_desc(COSMA_MODE_BF16X3, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+  return run(&d);
+}
+
+int cosma_num_tile_configs(int mode) {
+  if (mode < 0 || mode > 2) return 0;
+  return kNumCfg[mode];
+}
+
+int cosma_tile_config(int mode, int cfg, int* out7) {
+  if (!out7 || mode < 0 || mode > 2 || cfg < 0 || cfg >= kNumCfg[mode])
+    return COSMA_STATUS_INVALID_VALUE;
+  const cosma::TileInfo* t = cosma::tile_info(mode, cfg);
+  if (!t || t->BM != kDims[mode][cfg].bm || t->BN != kDims[mode][cfg].bn ||
+      t->BK != kDims[mode][cfg].bk)
+    return COSMA_STATUS_INVALID_VALUE;
+  out7[0] = t->BM;
+  out7[1] = t->BN;
+  out7[2] = t->BK;
+  out7[3] = t->WM;
+  out7[4] = t->WN;
+  out7[5] = t->threads;
+  out7[6] = t->lds_bytes;
+  return 0;
+}
+
+int cosma_heuristic(int mode, int m, int n, int k, int* tile_cfg, int* split_k) {
+  if (mode < 0 || mode > 2 || m < 0 || n < 0 || k < 0 || !tile_cfg || !split_k)
+    return COSMA_STATUS_INVALID_VALUE;
+  Choice ch = heuristic(mode, m, n, k);
+  *tile_cfg = ch.cfg;
+  *split_k = ch.split;
+  return 0;
+}
+
+int cosma_error_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
+                          int64_t ldr, double* out5, void* stream) {
+  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols)
+    return COSMA_STATUS_INVALID_VALUE;
+  double *part, *dout;
+  int st = stats_scratch(&part, &dout);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return finish_stats(cosma::run_err_stats_f32(rows, cols, C, ldc, R, ldr, part, dout, s), dout, out5, s);
+}
+int cosma_error_stats_f64(int rows, int cols, const double* C, int64_t ldc, const double* R,
+                          int64_t ldr, double* out5, void* stream) {
+  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols)
+    return COSMA_STATUS_INVALID_VALUE;
+  double *part, *dout;
+  int st = stats_scratch(&part, &dout);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return finish_stats(cosma::run_err_stats_f64(rows, cols, C, ldc, R, ldr, part, dout, s), dout, out5, s);
+}
+int cosma_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* out5, void* stream) {
+  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && !X) || ldx < cols)
+    return COSMA_STATUS_INVALID_VALUE;
+  double *part, *dout;
+  int st = stats_scratch(&part, &dout);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return finish_stats(cosma::run_sumsq_f32(rows, cols, X, ldx, part, dout, s), dout, out5, s);
+}
+int cosma_sumsq_f64(int rows, int cols, const double* X, int64_t ldx, double* out5, void* stream) {
+  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && !X) || ldx < cols)
+    return COSMA_STATUS_INVALID_VALUE;
+  double *part, *dout;
+  int st = stats_scratch(&part, &dout);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return finish_stats(cosma::run_sumsq_f64(rows, cols, X, ldx, part, dout, s), dout, out5, s);
+}
+int cosma_convert_f32_f64(int rows, int cols, const float* X, int64_t ldx, double* Y, int64_t ldy,
+                          void* stream) {
+  if (rows < 0 || cols < 0 || ((rows && cols) && (!X || !Y)) || ldx < cols || ldy < cols)
+    return COSMA_STATUS_INVALID_VALUE;
+  if (rows == 0 || cols == 0) return 0;
+  return (int)cosma::run_convert_f32_f64(rows, cols, X, ldx, Y, ldy,
+                                         static_cast<hipStream_t>(stream));
+}
+
+const char* cosma_status_string(int status) {
+  switch (status) {
+    case COSMA_STATUS_SUCCESS: return "success";
+    case COSMA_STATUS_INVALID_VALUE: return "invalid value (size, pointer, leading dimension, split or config)";
+    case COSMA_STATUS_MISALIGNED: return "operand breaks the 16-byte row alignment contract";
+    case COSMA_STATUS_NOT_SUPPORTED: return "combination not supported";
+    case COSMA_STATUS_WORKSPACE: return "split-K workspace unavailable while the stream is capturing";
+    default: return status > 0 ? hipGetErrorString((hipError_t)status) : "unknown status";
+  }
+}
+
+const char* cosma_version(void) { return "0.1.0"; }
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+// cosma_aux.hip -- split-K reduction, beta scaling, FP64 residual statistics
+// and the tile-configuration table.
+#include "cosma_internal.hpp"
+#include "cosma_kernels.hpp"
+
+namespace cosma {
+
+// Fixed-order split-K reduction: C = alpha * sum_{z=0..S-1} ws[z] (+ beta*C).
+// Every element sums its partials in ascending z, so the result does not
+// depend on launch order or timing (Kernel/cuda/src/cosmaSplitKReduce.cuh:20-42
+// sums in the same order).  beta == 0 does not read C.
+template <typename E>
+__global__ void __launch_bounds__(256)
+    splitk_reduce_kernel(const E* __restrict__ ws, E* __restrict__ C, int64_t ldc, int M, int N,
+                         int S, E alpha, E beta) {
+  const int64_t total = (int64_t)M * N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    E v = E(0);
+    for (int s = 0; s < S; ++s) v += ws[(int64_t)s * total + e];
+    const int row = (int)(e / N);
+    const int col = (int)(e - (int64_t)row * N);
+    E* cp = C + (int64_t)row * ldc + col;
+    E out = alpha * v;
+    if (beta != E(0)) out = fma_e(beta, *cp, out);
+    *cp = out;
+  }
+}
+
+// C = beta * C over the M x N window (alpha == 0 or K == 0); beta == 0 writes
+// zeros without reading C.
+template <typename E>
+__global__ void __launch_bounds__(256)
+    scale_kernel(E* __restrict__ C, int64_t ldc, int M, int N, E beta) {
+  const int64_t total = (int64_t)M * N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int row = (int)(e / N);
+    const int col = (int)(e - (int64_t)row * N);
+    E* cp = C + (int64_t)row * ldc + col;
+    *cp = beta == E(0) ? E(0) : beta * *cp;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual statistics on the GPU.  Deterministic: the grid size is fixed,
+// each thread walks a fixed element sequence, each block reduces in a fixed
+// tree order, and one block reduces the block partials in index order.
+// Statistics: [0] sum (C-R)^2, [1] sum R^2, [2] max |C-R|,
+// [3] max over R != 0 of |C-R|/|R|, [4] max |R|.
+// ---------------------------------------------------------------------------
+constexpr int kStatBlocks = 1024;
+constexpr int kStatThreads = 256;
+constexpr int kNStat = 5;
+
+__device__ __forceinline__ void stat_combine(double* a, const double* b) {
+  a[0] += b[0];
+  a[1] += b[1];
+  a[2] = fmax(a[2], b[2]);
+  a[3] = fmax(a[3], b[3]);
+  a[4] = fmax(a[4], b[4]);
+}
+
+__device__ __forceinline__ void block_stat_reduce(double (&mine)[kNStat], double* out) {
+  __shared__ double red[kStatThreads][kNStat];
+#pragma unroll
+  for (int s = 0; s < kNStat; ++s) red[threadIdx.x][s] = mine[s];
+  __syncthreads();
+  for (int w = kStatThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) stat_combine(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int s = 0; s < kNStat; ++s) out[s] = red[0][s];
+}
+
+template <typename TC>
+__global__ void __launch_bounds__(kStatThreads)
+    err_stats_partial(int rows, int cols, const TC* __restrict__ C, int64_t ldc,
+                      const double* __restrict__ R, int64_t ldr, double* __restrict__ part) {
+  double mine[kNStat] = {0, 0, 0, 0, 0};
+  const int64_t total = (int64_t)rows * cols;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int i = (int)(e / cols);
+    const int j = (int)(e - (int64_t)i * cols);
+    const double r = R[(int64_t)i * ldr + j];
+    const double d = fabs((double)C[(int64_t)i * ldc + j] - r);
+    mine[0] += d * d;
+    mine[1] += r * r;
+    mine[2] = fmax(mine[2], d);
+    if (r != 0.0) mine[3] = fmax(mine[3], d / fabs(r));
+    mine[4] = fmax(mine[4], fabs(r));
+  }
+  block_stat_reduce(mine, part + (int64_t)blockIdx.x * kNStat);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kStatThreads)
+    sumsq_partial(int rows, int cols, const T* __restrict__ X, int64_t ldx,
+                  double* __restrict__ part) {
+  double mine[kNStat] = {0, 0, 0, 0, 0};
+  const int64_t total = (int64_t)rows * cols;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int i = (int)(e / cols);
+    const int j = (int)(e - (int64_t)i * cols);
+    const double x = (double)X[(int64_t)i * ldx + j];
+    mine[0] += x * x;
+    mine[2] = fmax(mine[2], fabs(x));
+  }
+  block_stat_reduce(mine, part + (int64_t)blockIdx.x * kNStat);
+}
+
+__global__ void __launch_bounds__(kStatThreads)
+    stats_final(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+  double mine[kNStat] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) stat_combine(mine, part + (int64_t)b * kNStat);
+  block_stat_reduce(mine, out);
+}
+
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256)
+    convert_kernel(int rows, int cols, const TI* __restrict__ X, int64_t ldx, TO* __restrict__ Y,
+                   int64_t ldy) {
+  const int64_t total = (int64_t)rows * cols;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int i = (int)(e / cols);
+    const int j = (int)(e - (int64_t)i * cols);
+    Y[(int64_t)i * ldy + j] = (TO)X[(int64_t)i * ldx + j];
+  }
+}
+
+
+extern const TileInfo kF32Tiles[];
+extern const int kF32NumTiles;
+extern const TileInfo kF64Tiles[];
+extern const int kF64NumTiles;
+extern const TileInfo kBF16x3Tiles[];
+extern const int kBF16x3NumTiles;
+
+int num_tile_cfgs(int mode) {
+  switch (mode) {
+    case kF32: return kF32NumTiles;
+    case kF64: return kF64NumTiles;
+    case kBF16x3: return kBF16x3NumTiles;
+    default: return 0;
+  }
+}
+
+const TileInfo* tile_info(int mode, int cfg) {
+  if (cfg < 0 || cfg >= num_tile_cfgs(mode)) return nullptr;
+  switch (mode) {
+    case kF32: return &kF32Tiles[cfg];
+    case kF64: return &kF64Tiles[cfg];
+    default: return &kBF16x3Tiles[cfg];
+  }
+}
+
+static int grid_for(int64_t total, int threads, int cap) {
+  int64_t b = (total + threads - 1) / threads;
+  if (b < 1) b = 1;
+  return (int)(b < cap ? b : cap);
+}
+
+hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, hipStream_t s) {
+  hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for((int64_t)M * N, 256, 8192)), dim3(256), 0,
+                     s, C, ldc, M, N, beta);
+  return hipGetLastError();
+}
+hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, hipStream_t s) {
+  hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for((int64_t)M * N, 256, 8192)), dim3(256), 0,
+                     s, C, ldc, M, N, beta);
+  return hipGetLastError();
+}
+
+hipError_t run_reduce_f32(const float* ws, float* C, int64_t ldc, int M, int N, int S, float alpha,
+                          float beta, hipStream_t s) {
+  hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(grid_for((int64_t)M * N, 256, 8192)),
+                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta);
+  return hipGetLastError();
+}
+hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N, int S,
+                          double alpha, double beta, hipStream_t s) {
+  hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(grid_for((int64_t)M * N, 256, 8192)),
+                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta);
+  return hipGetLastError();
+}
+
+static_assert(kStatParts == kStatBlocks, "partial buffer size must match the stats grid");
+
+hipError_t run_err_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
+                             int64_t ldr, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(err_stats_partial<float>, dim3(kStatBlocks), dim3(kStatThreads), 0, s, rows,
+                     cols, C, ldc, R, ldr, part);
+  hipLaunchKernelGGL(stats_final, dim3(1), dim3(kStatThreads), 0, s, part, kStatBlocks, out);
+  return hipGetLastError();
+}
+hipError_t run_err_stats_f64(int rows, int cols, const double* C, int64_t ldc, const double* R,
+                             int64_t ldr, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(err_stats_partial<double>, dim3(kStatBlocks), dim3(kStatThreads), 0, s, rows,
+                     cols, C, ldc, R, ldr, part);
+  hipLaunchKernelGGL(stats_final, dim3(1), dim3(kStatThreads), 0, s, part, kStatBlocks, out);
+  return hipGetLastError();
+}
+hipError_t run_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* part, double* out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(sumsq_partial<float>, dim3(kStatBlocks), dim3(kStatThreads), 0, s, rows, cols,
+                     X, ldx, part);
+  hipLaunchKernelGGL(stats_final, dim3(1), dim3(kStatThreads), 0, s, part, kStatBlocks, out);
+  return hipGetLastError();
+}
+hipError_t run_sumsq_f64(int rows, int cols, const double* X, int64_t ldx, double* part,
+                         double* out, hipStream_t s) {
+  hipLaunchKernelGGL(sumsq_partial<double>, dim3(kStatBlocks), dim3(kStatThreads), 0, s, rows,
+                     cols, X, ldx, part);
+  hipLaunchKernelGGL(stats_final, dim3(1), dim3(kStatThreads), 0, s, part, kStatBlocks, out);
+  return hipGetLastError();
+}
+hipError_t run_convert_f32_f64(int rows, int cols, const float* X, int64_t ldx, double* Y,
+                               int64_t ldy, hipStream_t s) {
+  hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for((int64_t)rows * cols, 256, 8192)),
+                     dim3(256), 0, s, rows, cols, X, ldx, Y, ldy);
+  return hipGetLastError();
+}
+
+}  // namespace cosma

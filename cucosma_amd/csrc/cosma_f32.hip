@@ -1,0 +1,28 @@
+// cosma_f32.hip -- FP32 GEMM on v_mfma_f32_16x16x4_f32: tile configurations.
+#include "cosma_launch.hpp"
+
+namespace cosma {
+
+// cfg 0: 128x128 workgroup tile, 4 waves of 64x64 (the default for large shapes)
+// cfg 1: 64x64, 4 waves of 32x32 (small shapes, split-K slices)
+// cfg 2: 128x64, 4 waves of 64x32
+// cfg 3: 256x128, 8 waves of 64x64
+using F32C0 = Tile<kF32, 128, 128, 32, 64, 64>;
+using F32C1 = Tile<kF32, 64, 64, 32, 32, 32>;
+using F32C2 = Tile<kF32, 128, 64, 32, 64, 32>;
+using F32C3 = Tile<kF32, 256, 128, 32, 64, 64>;
+
+extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info()};
+extern const int kF32NumTiles = 4;
+
+hipError_t run_gemm_f32(const Problem& p) {
+  switch (p.cfg) {
+    case 0: return F32C0::run(p);
+    case 1: return F32C1::run(p);
+    case 2: return F32C2::run(p);
+    case 3: return F32C3::run(p);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace cosma

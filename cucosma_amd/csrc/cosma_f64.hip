@@ -1,0 +1,25 @@
+// cosma_f64.hip -- FP64 GEMM on v_mfma_f64_16x16x4_f64: tile configurations.
+#include "cosma_launch.hpp"
+
+namespace cosma {
+
+// cfg 0: 128x128 workgroup tile, 4 waves of 64x64
+// cfg 1: 64x64, 4 waves of 32x32
+// cfg 2: 128x64, 4 waves of 64x32
+using F64C0 = Tile<kF64, 128, 128, 16, 64, 64>;
+using F64C1 = Tile<kF64, 64, 64, 16, 32, 32>;
+using F64C2 = Tile<kF64, 128, 64, 16, 64, 32>;
+
+extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info()};
+extern const int kF64NumTiles = 3;
+
+hipError_t run_gemm_f64(const Problem& p) {
+  switch (p.cfg) {
+    case 0: return F64C0::run(p);
+    case 1: return F64C1::run(p);
+    case 2: return F64C2::run(p);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace cosma

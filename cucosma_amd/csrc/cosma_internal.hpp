@@ -1,0 +1,71 @@
+// cosma_internal.hpp -- host-side interface between the C ABI (cosma_api.cpp)
+// and the per-mode kernel launchers (cosma_f32.hip, cosma_f64.hip,
+// cosma_bf16x3.hip, cosma_aux.hip).  No HIP device code here.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace cosma {
+
+// One fully validated GEMM launch.  Shapes are non-negative, pointers are
+// non-null where used, leading dimensions and alignment satisfy the contract
+// checked in cosma_api.cpp, split >= 1, and ws holds split*M*N accumulator
+// elements whenever split > 1.
+struct Problem {
+  int mode;  // 0 fp32, 1 fp64, 2 bf16x3
+  int M, N, K;
+  int ta, tb;
+  const void* A;
+  int64_t lda;
+  const void* B;
+  int64_t ldb;
+  void* C;
+  int64_t ldc;
+  double alpha, beta;
+  int split;
+  int cfg;
+  int group_m;
+  void* ws;
+  hipStream_t stream;
+};
+
+// Tile configuration of one kernel instantiation family.
+struct TileInfo {
+  int BM, BN, BK, WM, WN;
+  int threads;
+  int lds_bytes;
+};
+
+int num_tile_cfgs(int mode);
+const TileInfo* tile_info(int mode, int cfg);  // nullptr when out of range
+
+// Launch the main kernel (and, when split > 1, the fixed-order reduction).
+hipError_t run_gemm_f32(const Problem& p);
+hipError_t run_gemm_f64(const Problem& p);
+hipError_t run_gemm_bf16x3(const Problem& p);
+
+// C = beta*C over the M x N window (alpha == 0 or K == 0).
+hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, hipStream_t s);
+hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, hipStream_t s);
+
+// Fixed-order split-K reduction.
+hipError_t run_reduce_f32(const float* ws, float* C, int64_t ldc, int M, int N, int S, float alpha,
+                          float beta, hipStream_t s);
+hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N, int S,
+                          double alpha, double beta, hipStream_t s);
+
+// FP64 statistics; part must hold kStatParts * 5 doubles, out 5 doubles (device).
+constexpr int kStatParts = 1024;
+hipError_t run_err_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
+                             int64_t ldr, double* part, double* out, hipStream_t s);
+hipError_t run_err_stats_f64(int rows, int cols, const double* C, int64_t ldc, const double* R,
+                             int64_t ldr, double* part, double* out, hipStream_t s);
+hipError_t run_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* part, double* out,
+                         hipStream_t s);
+hipError_t run_sumsq_f64(int rows, int cols, const double* X, int64_t ldx, double* part,
+                         double* out, hipStream_t s);
+hipError_t run_convert_f32_f64(int rows, int cols, const float* X, int64_t ldx, double* Y,
+                               int64_t ldy, hipStream_t s);
+
+}  // namespace cosma

@@ -1,0 +1,447 @@
+// cosma_kernels.hpp -- gfx950 device code of the COSMA-tiled GEMM.
+//
+// One kernel template covers three arithmetic modes that share the tiling:
+//   kF32     C = alpha*A*B + beta*C on v_mfma_f32_16x16x4_f32 (FP32 in, FP32 accumulate)
+//   kF64     the same on v_mfma_f64_16x16x4_f64
+//   kBF16x3  FP32 operands split on the fly into bf16 hi + lo parts while the
+//            tile is written to LDS; hi*hi + hi*lo + lo*hi on
+//            v_mfma_f32_16x16x32_bf16 with FP32 accumulation
+//
+// Reference behaviour this replaces (Kernel/cuda/src/cuCOSMAV100.cuh):
+//   threadblock / warp / thread tiles     :2505-2565   -> workgroup / wave / MFMA sub-tiles
+//   double-buffered shared-memory staging :2567-2581, 2658-2718 -> two LDS stages, one barrier per K chunk
+//   split-K over blockIdx.z               :59,125,298,2814-2831 -> K slice per split index, partials to a workspace
+//   split-K reduce                        Kernel/cuda/src/cosmaSplitKReduce.cuh:20-42 -> splitk_reduce_kernel
+//   ALPHA / BETA epilogue                 :1575-1595, 876-950  -> epilogue(): alpha*acc, then fma(beta, C, .)
+//   block swizzle                         :2583-2603  -> map_block(): XCD-aware remap + grouped tile order
+//
+// Layout conventions.  A is logically M x K and B is K x N, both addressed
+// through a leading dimension.  TA == false: A(i,k) = A[i*lda + k] (row-major,
+// the reference's only layout); TA == true: A(i,k) = A[k*lda + i].  TB ==
+// false: B(k,j) = B[k*ldb + j]; TB == true: B(k,j) = B[j*ldb + k].  In LDS both
+// operands are stored "k-contiguous": sA[i][k], sB[j][k], rows padded by 16
+// bytes so that the 16-byte fragment reads of 8 consecutive rows fall into
+// distinct bank groups.  An operand whose global layout is k-contiguous is
+// copied with 16-byte loads and stores; the other one is loaded as 4x4 (2x2
+// for FP64) element blocks and transposed in registers before its 16-byte LDS
+// stores.
+//
+// MFMA operand mapping (16x16 output sub-tile, 64 lanes, g = lane/16,
+// r16 = lane%16): lane supplies A(row r16, k-group g) and B(k-group g, col
+// r16); the accumulator holds C(4g + r, r16) in element r.  Each lane reads
+// KPL consecutive k of its row with one 16-byte LDS read, and step s of the
+// MFMA sequence consumes element s of that read for both operands, so A and B
+// always agree on which k an MFMA slot carries.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cosma {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+enum Mode : int { kF32 = 0, kF64 = 1, kBF16x3 = 2 };
+
+template <int MODE>
+struct Traits;
+
+template <>
+struct Traits<kF32> {
+  using G = float;    // element in global memory (A, B, C)
+  using S = float;    // element in LDS
+  using GV = f32x4;   // one 16-byte global vector
+  using Frag = f32x4; // one 16-byte LDS fragment read
+  using Acc = f32x4;  // accumulator of a 16x16 sub-tile, per lane
+  using E = float;    // accumulator / epilogue scalar
+  static constexpr int GVEC = 4;  // elements per global vector
+  static constexpr int KPL = 4;   // consecutive k per lane per fragment read
+  static constexpr int STEPS = 4; // MFMAs per fragment read
+  static constexpr int PAD = 4;   // LDS row padding (elements) = 16 bytes
+  static constexpr int PLANES = 1;
+};
+template <>
+struct Traits<kF64> {
+  using G = double;
+  using S = double;
+  using GV = f64x2;
+  using Frag = f64x2;
+  using Acc = f64x4;
+  using E = double;
+  static constexpr int GVEC = 2;
+  static constexpr int KPL = 2;
+  static constexpr int STEPS = 2;
+  static constexpr int PAD = 2;
+  static constexpr int PLANES = 1;
+};
+template <>
+struct Traits<kBF16x3> {
+  using G = float;
+  using S = __bf16;
+  using GV = f32x4;
+  using Frag = bf16x8;
+  using Acc = f32x4;
+  using E = float;
+  static constexpr int GVEC = 4;
+  static constexpr int KPL = 8;
+  static constexpr int STEPS = 1;
+  static constexpr int PAD = 8;
+  static constexpr int PLANES = 2;  // hi and lo
+};
+
+template <int MODE>
+struct GemmParams {
+  using G = typename Traits<MODE>::G;
+  using E = typename Traits<MODE>::E;
+  const G* A;
+  const G* B;
+  G* C;
+  E* ws;  // split-K partials [split][M][N]; unused when split == 1
+  int64_t lda, ldb, ldc;
+  int M, N, K;
+  int tiles_m, tiles_n;
+  int split;   // number of K slices
+  int kslice;  // K per slice, a multiple of the tile's BK
+  int group_m; // tile rows per group in the grouped tile order
+  E alpha, beta;
+};
+
+// Split four floats into RNE bf16 hi parts and RNE bf16 lo = x - hi parts
+// (x - hi is exact in FP32).  v_cvt_pk_bf16_f32 does the rounding.
+__device__ __forceinline__ void split_bf16x4(f32x4 v, bf16x4& hi, bf16x4& lo) {
+  f32x2 v01 = {v[0], v[1]};
+  f32x2 v23 = {v[2], v[3]};
+  bf16x2 h01 = __builtin_convertvector(v01, bf16x2);
+  bf16x2 h23 = __builtin_convertvector(v23, bf16x2);
+  f32x2 r01 = v01 - __builtin_convertvector(h01, f32x2);
+  f32x2 r23 = v23 - __builtin_convertvector(h23, f32x2);
+  bf16x2 l01 = __builtin_convertvector(r01, bf16x2);
+  bf16x2 l23 = __builtin_convertvector(r23, bf16x2);
+  hi = bf16x4{h01[0], h01[1], h23[0], h23[1]};
+  lo = bf16x4{l01[0], l01[1], l23[0], l23[1]};
+}
+
+// Linear block id -> (tile row, tile col, split index).
+// Workgroups are handed to the 8 XCDs round-robin by id, so ids b, b+8, ...
+// share an L2.  The first remap gives each XCD a contiguous range of logical
+// ids; the grouped order then walks group_m tile rows column by column, so the
+// workgroups resident on one XCD at a time share A row panels and B column
+// panels.  Split index is the slowest-varying coordinate.
+__device__ __forceinline__ void map_block(int bid, int nblk, int tiles_m, int tiles_n,
+                                          int group_m, int& tm, int& tn, int& z) {
+  constexpr int kXcd = 8;
+  const int full = (nblk / kXcd) * kXcd;
+  int lid = bid;
+  if (bid < full) lid = (bid % kXcd) * (full / kXcd) + bid / kXcd;
+  const int tiles = tiles_m * tiles_n;
+  z = lid / tiles;
+  const int t = lid - z * tiles;
+  const int per_group = group_m * tiles_n;
+  const int grp = t / per_group;
+  const int first_m = grp * group_m;
+  const int gsz = min(tiles_m - first_m, group_m);
+  const int r = t - grp * per_group;
+  tm = first_m + r % gsz;
+  tn = r / gsz;
+}
+
+// Global -> register -> LDS copy of one operand tile of R rows (M or N
+// extent) by BK (k extent).  KMAJ: global memory is k-contiguous for a row.
+template <int MODE, int R, int BK, int NT, bool KMAJ, bool EDGE>
+struct TileLoader {
+  using T = Traits<MODE>;
+  using G = typename T::G;
+  using GV = typename T::GV;
+  using S = typename T::S;
+  static constexpr int GVEC = T::GVEC;
+  static constexpr int LDK = BK + T::PAD;
+  // KMAJ: units are 16-byte vectors along k.  Otherwise units are GVEC x GVEC
+  // blocks (GVEC k rows of one GVEC-wide vector along the row index).
+  static constexpr int UNITS = KMAJ ? (R * BK / GVEC) : ((R / GVEC) * (BK / GVEC));
+  static constexpr int PER = (UNITS + NT - 1) / NT;
+  static constexpr int NREG = KMAJ ? PER : PER * GVEC;
+  // When the tile has fewer units than threads, the surplus threads idle.
+  __device__ __forceinline__ static bool active(int idx) { return UNITS % NT == 0 || idx < UNITS; }
+  static_assert(BK % GVEC == 0 && R % GVEC == 0, "tile extents must be whole vectors");
+
+  GV v[NREG];
+
+  __device__ __forceinline__ static GV load_vec(const G* p, bool row_ok, int e_valid) {
+    if (!EDGE || (row_ok && e_valid >= GVEC)) return *reinterpret_cast<const GV*>(p);
+    GV out;
+#pragma unroll
+    for (int e = 0; e < GVEC; ++e) out[e] = (row_ok && e < e_valid) ? p[e] : G(0);
+    return out;
+  }
+
+  // base: operand pointer; ld: its leading dimension; r0: first tile row;
+  // rmax: rows in the problem (M or N); k0: first k of the chunk; kend: end of
+  // the valid k range of this split.
+  __device__ __forceinline__ void load(const G* __restrict__ base, int64_t ld, int r0, int rmax,
+                                       int k0, int kend, int tid) {
+    if constexpr (KMAJ) {
+      constexpr int VPR = BK / GVEC;  // vectors per row
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int idx = tid + i * NT;
+        if (!active(idx)) continue;
+        const int r = idx / VPR;
+        const int kv = idx - r * VPR;
+        const int gr = r0 + r;
+        const int gk = k0 + kv * GVEC;
+        const G* p = base + (int64_t)gr * ld + gk;
+        v[i] = load_vec(p, gr < rmax, kend - gk);
+      }
+    } else {
+      constexpr int RB = R / GVEC;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int idx = tid + i * NT;
+        if (!active(idx)) continue;
+        const int kb = idx / RB;
+        const int rb = idx - kb * RB;
+        const int gr = r0 + rb * GVEC;
+#pragma unroll
+        for (int u = 0; u < GVEC; ++u) {
+          const int gk = k0 + kb * GVEC + u;
+          const G* p = base + (int64_t)gk * ld + gr;
+          v[i * GVEC + u] = load_vec(p, gk < kend, rmax - gr);
+        }
+      }
+    }
+  }
+
+  // Writes the registers to LDS plane(s) sh (and sl for the bf16 lo part).
+  __device__ __forceinline__ void store(S* __restrict__ sh, S* __restrict__ sl, int tid) const {
+    if constexpr (KMAJ) {
+      constexpr int VPR = BK / GVEC;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int idx = tid + i * NT;
+        if (!active(idx)) continue;
+        const int r = idx / VPR;
+        const int kv = idx - r * VPR;
+        put(sh, sl, r * LDK + kv * GVEC, v[i]);
+      }
+    } else {
+      constexpr int RB = R / GVEC;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int idx = tid + i * NT;
+        if (!active(idx)) continue;
+        const int kb = idx / RB;
+        const int rb = idx - kb * RB;
+#pragma unroll
+        for (int t = 0; t < GVEC; ++t) {
+          GV w;
+#pragma unroll
+          for (int u = 0; u < GVEC; ++u) w[u] = v[i * GVEC + u][t];
+          put(sh, sl, (rb * GVEC + t) * LDK + kb * GVEC, w);
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ static void put(S* sh, S* sl, int off, GV w) {
+    if constexpr (MODE == kBF16x3) {
+      bf16x4 h, l;
+      split_bf16x4(w, h, l);
+      *reinterpret_cast<bf16x4*>(sh + off) = h;
+      *reinterpret_cast<bf16x4*>(sl + off) = l;
+    } else {
+      (void)sl;
+      *reinterpret_cast<GV*>(sh + off) = w;
+    }
+  }
+};
+
+__device__ __forceinline__ float fma_e(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_e(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <int MODE>
+__device__ __forceinline__ typename Traits<MODE>::Acc zero_acc() {
+  typename Traits<MODE>::Acc a;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) a[r] = 0;
+  return a;
+}
+
+// Epilogue of one 16x16 sub-tile element set: split == 1 writes
+// alpha*acc (+ beta*C when beta != 0; C is not read when beta == 0);
+// split > 1 writes the raw partial to ws[z][row][col].
+template <int MODE, bool EDGE>
+__device__ __forceinline__ void epilogue_store(const GemmParams<MODE>& p, int z, int row0, int col,
+                                               const typename Traits<MODE>::Acc& acc) {
+  using E = typename Traits<MODE>::E;
+  if (EDGE && col >= p.N) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = row0 + r;
+    if (EDGE && row >= p.M) continue;
+    if (p.split == 1) {
+      E v = p.alpha * acc[r];
+      E* cp = p.C + (int64_t)row * p.ldc + col;
+      if (p.beta != E(0)) v = fma_e(p.beta, *cp, v);
+      *cp = v;
+    } else {
+      p.ws[((int64_t)z * p.M + row) * p.N + col] = acc[r];
+    }
+  }
+}
+
+template <int MODE, int BM, int BN, int BK, int WM, int WN>
+struct TileShape {
+  static constexpr int WAVES_M = BM / WM;
+  static constexpr int WAVES_N = BN / WN;
+  static constexpr int NT = 64 * WAVES_M * WAVES_N;
+  static constexpr int TM = WM / 16;
+  static constexpr int TN = WN / 16;
+  static constexpr int LDK = BK + Traits<MODE>::PAD;
+  static constexpr int KR = 4 * Traits<MODE>::KPL;  // k covered per fragment read round
+  static constexpr int PLANE = (BM + BN) * LDK;      // elements of one hi or lo plane
+  static constexpr int STAGE = PLANE * Traits<MODE>::PLANES;
+  static_assert(BM % WM == 0 && BN % WN == 0 && WM % 16 == 0 && WN % 16 == 0, "wave tiling");
+  static_assert(BK % KR == 0, "BK must cover whole fragment rounds");
+};
+
+// MFMA sequence over one LDS stage for one wave's TM x TN sub-tiles.
+template <int MODE, int BM, int BN, int BK, int WM, int WN>
+__device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __restrict__ st,
+                                              typename Traits<MODE>::Acc (&acc)[WM / 16][WN / 16],
+                                              int wm0, int wn0, int lane) {
+  using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+  using T = Traits<MODE>;
+  using Frag = typename T::Frag;
+  constexpr int TM = Sh::TM, TN = Sh::TN, LDK = Sh::LDK;
+  const int g = lane >> 4;
+  const int r16 = lane & 15;
+  const typename T::S* sa = st;
+  const typename T::S* sb = st + BM * LDK;
+#pragma unroll
+  for (int q = 0; q < BK / Sh::KR; ++q) {
+    const int kof = q * Sh::KR + g * T::KPL;
+    if constexpr (MODE == kBF16x3) {
+      const typename T::S* sal = st + Sh::PLANE;
+      const typename T::S* sbl = sal + BM * LDK;
+      Frag ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int off = (wm0 + i * 16 + r16) * LDK + kof;
+        ah[i] = *reinterpret_cast<const Frag*>(sa + off);
+        al[i] = *reinterpret_cast<const Frag*>(sal + off);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int off = (wn0 + j * 16 + r16) * LDK + kof;
+        bh[j] = *reinterpret_cast<const Frag*>(sb + off);
+        bl[j] = *reinterpret_cast<const Frag*>(sbl + off);
+      }
+      // Small cross terms first, then the leading hi*hi term.
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    } else {
+      Frag a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        a[i] = *reinterpret_cast<const Frag*>(sa + (wm0 + i * 16 + r16) * LDK + kof);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        b[j] = *reinterpret_cast<const Frag*>(sb + (wn0 + j * 16 + r16) * LDK + kof);
+#pragma unroll
+      for (int s = 0; s < T::STEPS; ++s)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            if constexpr (MODE == kF32)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+          }
+    }
+  }
+}
+
+// The GEMM kernel.  Grid: tiles_m * tiles_n * split workgroups (1-D).
+// EDGE == false requires M % BM == 0, N % BN == 0 and K % BK == 0.
+template <int MODE, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, bool EDGE>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
+    gemm_kernel(GemmParams<MODE> p) {
+  using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+  using T = Traits<MODE>;
+  using S = typename T::S;
+  constexpr int NT = Sh::NT;
+  constexpr int LDK = Sh::LDK;
+  __shared__ __attribute__((aligned(16))) S lds[2 * Sh::STAGE];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave / Sh::WAVES_N) * WM;
+  const int wn0 = (wave % Sh::WAVES_N) * WN;
+
+  int tm, tn, z;
+  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
+  const int kbeg = z * p.kslice;
+  const int kend = min(p.K, kbeg + p.kslice);
+  const int nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+
+  TileLoader<MODE, BM, BK, NT, !TA, EDGE> la;
+  TileLoader<MODE, BN, BK, NT, TB, EDGE> lb;
+
+  typename T::Acc acc[Sh::TM][Sh::TN];
+#pragma unroll
+  for (int i = 0; i < Sh::TM; ++i)
+#pragma unroll
+    for (int j = 0; j < Sh::TN; ++j) acc[i][j] = zero_acc<MODE>();
+
+  auto stage_ptr = [&](int s) { return lds + s * Sh::STAGE; };
+  auto store_stage = [&](int s) {
+    S* st = stage_ptr(s);
+    la.store(st, st + Sh::PLANE, tid);
+    lb.store(st + BM * LDK, st + Sh::PLANE + BM * LDK, tid);
+  };
+
+  if (nk > 0) {
+    la.load(p.A, p.lda, m0, p.M, kbeg, kend, tid);
+    lb.load(p.B, p.ldb, n0, p.N, kbeg, kend, tid);
+    store_stage(0);
+    __syncthreads();
+  }
+  for (int it = 0; it < nk; ++it) {
+    const int cur = it & 1;
+    const bool more = it + 1 < nk;
+    if (more) {
+      const int k0 = kbeg + (it + 1) * BK;
+      la.load(p.A, p.lda, m0, p.M, k0, kend, tid);
+      lb.load(p.B, p.ldb, n0, p.N, k0, kend, tid);
+    }
+    compute_stage<MODE, BM, BN, BK, WM, WN>(stage_ptr(cur), acc, wm0, wn0, lane);
+    if (more) store_stage(cur ^ 1);
+    __syncthreads();
+  }
+
+  const int g = lane >> 4;
+  const int r16 = lane & 15;
+#pragma unroll
+  for (int i = 0; i < Sh::TM; ++i)
+#pragma unroll
+    for (int j = 0; j < Sh::TN; ++j)
+      epilogue_store<MODE, EDGE>(p, z, m0 + wm0 + i * 16 + g * 4, n0 + wn0 + j * 16 + r16,
+                                 acc[i][j]);
+}
+
+}  // namespace cosma

@@ -1,0 +1,69 @@
+// cosma_launch.hpp -- instantiation and launch of gemm_kernel for one tile
+// configuration.  Included by the per-mode translation units only.
+#pragma once
+#include "cosma_internal.hpp"
+#include "cosma_kernels.hpp"
+
+namespace cosma {
+
+template <int MODE, int BM, int BN, int BK, int WM, int WN>
+struct Tile {
+  using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+  static constexpr TileInfo info() {
+    return TileInfo{BM, BN, BK, WM, WN, Sh::NT,
+                    (int)(2 * Sh::STAGE * sizeof(typename Traits<MODE>::S))};
+  }
+
+  template <bool TA, bool TB, bool EDGE>
+  static hipError_t go(const GemmParams<MODE>& gp, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL((gemm_kernel<MODE, BM, BN, BK, WM, WN, TA, TB, EDGE>), dim3(nblk),
+                       dim3(Sh::NT), 0, s, gp);
+    return hipGetLastError();
+  }
+
+  static hipError_t run(const Problem& pr) {
+    using G = typename Traits<MODE>::G;
+    using E = typename Traits<MODE>::E;
+    GemmParams<MODE> gp;
+    gp.A = static_cast<const G*>(pr.A);
+    gp.B = static_cast<const G*>(pr.B);
+    gp.C = static_cast<G*>(pr.C);
+    gp.ws = static_cast<E*>(pr.ws);
+    gp.lda = pr.lda;
+    gp.ldb = pr.ldb;
+    gp.ldc = pr.ldc;
+    gp.M = pr.M;
+    gp.N = pr.N;
+    gp.K = pr.K;
+    gp.tiles_m = (pr.M + BM - 1) / BM;
+    gp.tiles_n = (pr.N + BN - 1) / BN;
+    gp.split = pr.split;
+    const int per = (pr.K + pr.split - 1) / pr.split;
+    gp.kslice = ((per + BK - 1) / BK) * BK;
+    gp.group_m = pr.group_m > 0 ? pr.group_m : 8;
+    gp.alpha = (E)pr.alpha;
+    gp.beta = (E)pr.beta;
+    const int64_t nblk64 = (int64_t)gp.tiles_m * gp.tiles_n * pr.split;
+    if (nblk64 <= 0 || nblk64 > 0x7fffffff) return hipErrorInvalidValue;
+    const int nblk = (int)nblk64;
+    const bool edge = (pr.M % BM) || (pr.N % BN) || (pr.K % BK);
+    hipError_t e;
+    if (pr.ta == 0 && pr.tb == 0)
+      e = edge ? go<false, false, true>(gp, nblk, pr.stream) : go<false, false, false>(gp, nblk, pr.stream);
+    else if (pr.ta == 0 && pr.tb == 1)
+      e = edge ? go<false, true, true>(gp, nblk, pr.stream) : go<false, true, false>(gp, nblk, pr.stream);
+    else if (pr.ta == 1 && pr.tb == 0)
+      e = edge ? go<true, false, true>(gp, nblk, pr.stream) : go<true, false, false>(gp, nblk, pr.stream);
+    else
+      return hipErrorNotSupported;
+    if (e != hipSuccess || pr.split == 1) return e;
+    if constexpr (MODE == kF64)
+      return run_reduce_f64(static_cast<const double*>(pr.ws), static_cast<double*>(pr.C), pr.ldc,
+                            pr.M, pr.N, pr.split, pr.alpha, pr.beta, pr.stream);
+    else
+      return run_reduce_f32(static_cast<const float*>(pr.ws), static_cast<float*>(pr.C), pr.ldc,
+                            pr.M, pr.N, pr.split, (float)pr.alpha, (float)pr.beta, pr.stream);
+  }
+};
+
+}  // namespace cosma

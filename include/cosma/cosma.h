@@ -1,0 +1,182 @@
+/*
+ * cosma.h -- C ABI of the MI355X (gfx950) COSMA-tiled GEMM library
+ * (libcosma_amd.so).
+ *
+ * Drop-in boundary.  The reference exposes one call per build,
+ *
+ *     cublasStatus_t cosmaSgemm(const TYPE* A, int lda, const TYPE* B, int ldb,
+ *                               TYPE* C, int ldc);
+ *
+ * (Kernel/cuda/src/cuCOSMAV100.cuh:2774-2776; the HIP copy returns bool,
+ * Kernel/hip/src/cuCOSMAV100.cuh:2787-2789) with M, N, K, ALPHA, BETA and the
+ * tile sizes fixed by the macros of a generated config.h
+ * (Kernel/cuda/src/config.h:9-31).  C = ALPHA*A*B + BETA*C, all operands
+ * row-major device buffers owned by the caller, ld* in elements.
+ *
+ * This header keeps that contract (row-major, caller-owned device buffers,
+ * leading dimensions in elements, C = alpha*A*B + beta*C) and makes alpha and
+ * beta arguments.  Differences from the reference, all deliberate:
+ *   - beta == 0 never reads C (BLAS convention; the reference's split-K
+ *     reduce reads C even then, Kernel/cuda/src/cosmaSplitKReduce.cuh:38-40).
+ *   - alpha == 0 scales only the M x N window of C by beta (the reference
+ *     scales ldc*M elements, padding included, cuCOSMAV100.cuh:2782-2796).
+ *   - Split-K partials are always reduced in a fixed order; there is no
+ *     atomic variant, so results are bitwise repeatable.
+ *   - Errors are returned, never abort the process.
+ *   - Calls take an optional stream; NULL is the default stream.
+ *
+ * Status codes: 0 success; negative = contract violation detected on the host
+ * before any GPU work; positive = the hipError_t reported by the HIP runtime.
+ *
+ * Operand contract: every operand base pointer is 16-byte aligned and every
+ * leading dimension of A and B is a multiple of 4 elements (FP32) or 2
+ * elements (FP64), so each row starts on a 16-byte boundary -- the implicit
+ * float4 contract of the reference's loaders (cuCOSMAV100.cuh:143-144,
+ * 315-317), checked here.  C has no alignment requirement.
+ */
+#ifndef COSMA_COSMA_H_
+#define COSMA_COSMA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSMA_STATUS_SUCCESS 0
+#define COSMA_STATUS_INVALID_VALUE (-1) /* negative size, null pointer, ld too small, bad split/config */
+#define COSMA_STATUS_MISALIGNED (-2)    /* base not 16-byte aligned or ld breaks 16-byte rows */
+#define COSMA_STATUS_NOT_SUPPORTED (-3) /* combination not built (e.g. both operands transposed) */
+#define COSMA_STATUS_WORKSPACE (-4)     /* split-K workspace missing while the stream is capturing */
+
+#define COSMA_MODE_FP32 0   /* FP32 in, FP32 MFMA, FP32 out */
+#define COSMA_MODE_FP64 1   /* FP64 in, FP64 MFMA, FP64 out */
+#define COSMA_MODE_BF16X3 2 /* FP32 in, bf16 hi/lo split, 3 bf16 MFMA products, FP32 out */
+
+/* Full description of one GEMM: C = alpha * op(A) * op(B) + beta * C with
+ * op(A) M x K and op(B) K x N.  trans_a == 0: A(i,k) = a[i*lda + k];
+ * trans_a == 1: A(i,k) = a[k*lda + i].  trans_b == 0: B(k,j) = b[k*ldb + j];
+ * trans_b == 1: B(k,j) = b[j*ldb + k].  trans_a == trans_b == 1 is not built.
+ * Element type: float for FP32 and BF16X3, double for FP64. */
+typedef struct cosma_gemm_desc {
+    int mode;
+    int m, n, k;
+    int trans_a, trans_b;
+    const void *a;
+    int64_t lda;
+    const void *b;
+    int64_t ldb;
+    void *c;
+    int64_t ldc;
+    double alpha, beta;
+    int split_k;            /* <= 0: library heuristic; else 1..4096 */
+    int tile_cfg;           /* < 0: library heuristic; else index < cosma_num_tile_configs(mode) */
+    int group_m;            /* <= 0: default (8) tile rows per grouped-order group */
+    void *workspace;        /* split-K partials; NULL: library-owned buffer */
+    size_t workspace_bytes; /* size of workspace */
+    void *stream;           /* hipStream_t; NULL: default stream */
+} cosma_gemm_desc;
+
+/* Fill *d with defaults (heuristic split and tile, default stream). */
+void cosma_gemm_desc_init(cosma_gemm_desc *d);
+
+/* Run the GEMM described by *d.  Replaces the body of cosmaSgemm
+ * (Kernel/cuda/src/cuCOSMAV100.cuh:2774-2848) for every mode and layout. */
+int cosma_gemm_ex(const cosma_gemm_desc *d);
+
+/* Bytes of split-K workspace the call *d needs (0 when split_k resolves to 1). */
+int cosma_workspace_query(const cosma_gemm_desc *d, size_t *bytes);
+
+/* Install a caller-owned device buffer used for split-K partials by calls
+ * whose descriptor brings no workspace.  NULL/0 reverts to the library-owned
+ * buffer.  The reference allocates and frees the partial buffer inside every
+ * call (cuCOSMAV100.cuh:2826-2838); here it is allocated at most once, never
+ * while the stream is capturing.  Calls that share a workspace must be
+ * ordered on one stream. */
+int cosma_workspace_set(void *ptr, size_t bytes);
+
+/* Fixed-shape entry points: the shape is fixed when the library is built
+ * (COSMA_M, COSMA_N, COSMA_K; default 1024 x 1024 x 1024), as the reference
+ * fixes it in config.h.  Same operand order as cosmaSgemm, with alpha/beta
+ * passed instead of taken from ALPHA/BETA macros.  Default stream. */
+int cosma_sgemm(float alpha, const float *A, int lda, const float *B, int ldb, float beta,
+                float *C, int ldc);
+int cosma_dgemm(double alpha, const double *A, int lda, const double *B, int ldb, double beta,
+                double *C, int ldc);
+/* Shape the fixed-shape entry points were built for. */
+void cosma_fixed_shape(int *m, int *n, int *k);
+
+/* Compile-time specialisations for the benchmark shapes: the shape, tile
+ * configuration and split-K factor are constants of the instantiation, as one
+ * reference build per config.h row (Kernel/cuda/run.sh:185-257). */
+int cosma_sgemm_1024x1024x1024(float alpha, const float *A, int lda, const float *B, int ldb,
+                               float beta, float *C, int ldc);
+int cosma_sgemm_8192x8192x8192(float alpha, const float *A, int lda, const float *B, int ldb,
+                               float beta, float *C, int ldc);
+int cosma_sgemm_256x256x1048576(float alpha, const float *A, int lda, const float *B, int ldb,
+                                float beta, float *C, int ldc);
+int cosma_sgemm_bf16x3_8192x8192x8192(float alpha, const float *A, int lda, const float *B,
+                                      int ldb, float beta, float *C, int ldc);
+int cosma_sgemm_bf16x3_256x256x1048576(float alpha, const float *A, int lda, const float *B,
+                                       int ldb, float beta, float *C, int ldc);
+int cosma_dgemm_32768x1024x1024(double alpha, const double *A, int lda, const double *B, int ldb,
+                                double beta, double *C, int ldc);
+int cosma_dgemm_1024x65536x1024(double alpha, const double *A, int lda, const double *B, int ldb,
+                                double beta, double *C, int ldc);
+
+/* Runtime-shape variants (row-major, no transposes), on an explicit stream. */
+int cosma_sgemm_mnk(int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                    int ldb, float beta, float *C, int ldc, void *stream);
+int cosma_dgemm_mnk(int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                    int ldb, double beta, double *C, int ldc, void *stream);
+int cosma_sgemm_bf16x3_mnk(int m, int n, int k, float alpha, const float *A, int lda,
+                           const float *B, int ldb, float beta, float *C, int ldc, void *stream);
+
+/* Tile configurations built into the library.  out[0..6] = BM, BN, BK, WM,
+ * WN, threads per workgroup, LDS bytes.  Replaces the tile macros of
+ * config.h (THREADBLOCK_TILE_*, WARP_TILE_*, LOAD_K). */
+int cosma_num_tile_configs(int mode);
+int cosma_tile_config(int mode, int cfg, int *out7);
+/* The (tile_cfg, split_k) the library heuristic picks for a shape. */
+int cosma_heuristic(int mode, int m, int n, int k, int *tile_cfg, int *split_k);
+
+/* FP64 residual statistics computed on the GPU with a deterministic
+ * reduction.  out5: [0] sum (C-R)^2, [1] sum R^2, [2] max |C-R|,
+ * [3] max over R != 0 of |C-R|/|R|, [4] max |R|.  C is float (f32) or
+ * double (f64), R double; rows x cols windows with leading dimensions.
+ * Synchronises the stream and writes out5 on the host. */
+int cosma_error_stats_f32(int rows, int cols, const float *C, int64_t ldc, const double *R,
+                          int64_t ldr, double *out5, void *stream);
+int cosma_error_stats_f64(int rows, int cols, const double *C, int64_t ldc, const double *R,
+                          int64_t ldr, double *out5, void *stream);
+/* out5[0] = sum x^2 (FP64), out5[2] = max |x|; others 0.  Synchronises. */
+int cosma_sumsq_f32(int rows, int cols, const float *X, int64_t ldx, double *out5, void *stream);
+int cosma_sumsq_f64(int rows, int cols, const double *X, int64_t ldx, double *out5, void *stream);
+/* Y = (double)X, element-wise over a rows x cols window. */
+int cosma_convert_f32_f64(int rows, int cols, const float *X, int64_t ldx, double *Y, int64_t ldy,
+                          void *stream);
+
+/* Human-readable text for a status code (static storage). */
+const char *cosma_status_string(int status);
+/* Library version, e.g. "0.1.0". */
+const char *cosma_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+/* Literal drop-in for the reference call site: a translation unit that
+ * defines the reference's config.h macros M, N, K, ALPHA and BETA before
+ * including this header, and COSMA_REFERENCE_MACROS, can keep
+ * `cosmaSgemm(A, lda, B, ldb, C, ldc)` unchanged (test_correctness.h:96,
+ * benchmark.h:216).  The return value is 0 on success instead of
+ * CUBLAS_STATUS_SUCCESS. */
+#if defined(COSMA_REFERENCE_MACROS) && defined(M) && defined(N) && defined(K) && \
+    defined(ALPHA) && defined(BETA)
+#define cosmaSgemm(A_, lda_, B_, ldb_, C_, ldc_)                                                  \
+    cosma_sgemm_mnk((M), (N), (K), (float)(ALPHA), (A_), (lda_), (B_), (ldb_), (float)(BETA), \
+                    (C_), (ldc_), (void *)0)
+#endif
+
+#endif /* COSMA_COSMA_H_ */

@@ -1,0 +1,150 @@
+"""The C ABI without a GPU: the library loads, exports every function that
+include/cosma/cosma.h declares, and rejects contract violations on the host
+before any HIP call (so these run on a CPU-only machine)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cosma", "cosma.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cosma_[a-z0-9_x]+)\s*\(", text)) - {"cosma_sgemm_mnk_"})
+
+
+def test_library_exports_every_declared_symbol():
+    import cucosma_amd
+
+    path = cucosma_amd._lib.LIB_PATH
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (cosma_\w+)", out))
+    declared = set(_declared())
+    # the cosmaSgemm drop-in macro expands to cosma_sgemm_mnk, declared above
+    assert declared, "header parse found nothing"
+    missing = declared - exported
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+    for name in declared:
+        getattr(cucosma_amd.lib, name)
+
+
+def test_version_and_status_strings():
+    import cucosma_amd
+
+    assert cucosma_amd.__version__.count(".") == 2
+    for st in (0, -1, -2, -3, -4):
+        assert cucosma_amd.lib.cosma_status_string(st)
+
+
+def test_tile_table_and_heuristic():
+    import cucosma_amd
+
+    for mode in range(3):
+        cfgs = cucosma_amd.tile_configs(mode)
+        assert cfgs
+        for c in cfgs:
+            assert c["BM"] % c["WM"] == 0 and c["BN"] % c["WN"] == 0
+            assert c["threads"] == 64 * (c["BM"] // c["WM"]) * (c["BN"] // c["WN"])
+            assert c["lds_bytes"] <= 160 * 1024
+    assert cucosma_amd.heuristic(0, 8192, 8192, 8192) == (0, 1)
+    cfg, split = cucosma_amd.heuristic(0, 256, 256, 1 << 20)
+    assert 8 <= split <= 64
+    assert cucosma_amd.heuristic(1, 32768, 1024, 1024)[1] == 1
+    assert cucosma_amd.workspace_bytes(256, 256, 1 << 20) == split * 256 * 256 * 4
+    assert cucosma_amd.workspace_bytes(256, 256, 1 << 20, split_k=1) == 0
+
+
+def _desc(**kw):
+    import cucosma_amd
+    from cucosma_amd._lib import GemmDesc
+
+    d = GemmDesc()
+    cucosma_amd.lib.cosma_gemm_desc_init(C.byref(d))
+    d.m = d.n = d.k = 64
+    d.a, d.b, d.c = 0x1000, 0x2000, 0x3000
+    d.lda = d.ldb = d.ldc = 64
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize(
+    "kw,status",
+    [
+        (dict(m=-1), -1),
+        (dict(k=-5), -1),
+        (dict(mode=7), -1),
+        (dict(c=None), -1),
+        (dict(a=None), -1),
+        (dict(lda=63), -1),
+        (dict(ldb=10), -1),
+        (dict(ldc=3), -1),
+        (dict(trans_a=2), -1),
+        (dict(split_k=5000), -1),
+        (dict(tile_cfg=99), -1),
+        (dict(a=0x1004), -2),
+        (dict(lda=66), -2),
+        (dict(mode=1, lda=65), -2),
+        (dict(trans_a=1, trans_b=1), -3),
+    ],
+)
+def test_contract_violations_rejected_on_host(kw, status):
+    import cucosma_amd
+
+    d = _desc(**kw)
+    assert cucosma_amd.lib.cosma_gemm_ex(C.byref(d)) == status
+    size = C.c_size_t()
+    assert cucosma_amd.lib.cosma_workspace_query(C.byref(d), C.byref(size)) == status
+
+
+def test_empty_problems_are_noops_on_host():
+    import cucosma_amd
+
+    for kw in (dict(m=0), dict(n=0), dict(m=0, c=None)):
+        d = _desc(**kw)
+        assert cucosma_amd.lib.cosma_gemm_ex(C.byref(d)) == 0
+    # runtime-shape and fixed-shape entry points share the checks
+    lib = cucosma_amd.lib
+    assert lib.cosma_sgemm_mnk(-1, 4, 4, 1.0, 0x1000, 4, 0x1000, 4, 0.0, 0x1000, 4, None) == -1
+    assert lib.cosma_sgemm_mnk(4, 4, 4, 1.0, 0x1000, 4, 0x1000, 4, 0.0, None, 4, None) == -1
+    assert lib.cosma_sgemm_bf16x3_mnk(4, 4, 4, 1.0, 0x1001, 4, 0x1000, 4, 0.0, 0x1000, 4, None) == -2
+    assert lib.cosma_dgemm_mnk(4, 4, 4, 1.0, 0x1000, 3, 0x1000, 4, 0.0, 0x1000, 4, None) == -1
+    assert lib.cosma_sgemm_8192x8192x8192(1.0, 0x1000, 100, 0x1000, 8192, 0.0, 0x1000, 8192) == -1
+    assert lib.cosma_dgemm_32768x1024x1024(1.0, 0x1008, 1024, 0x1000, 1024, 0.0, 0x1000, 1024) == -2
+    m, n, k = C.c_int(), C.c_int(), C.c_int()
+    lib.cosma_fixed_shape(C.byref(m), C.byref(n), C.byref(k))
+    assert (m.value, n.value, k.value) == (1024, 1024, 1024)
+    assert lib.cosma_error_stats_f32(-1, 2, None, 2, None, 2, (C.c_double * 5)(), None) == -1
+    assert lib.cosma_convert_f32_f64(2, 2, None, 2, None, 2, None) == -1
+
+
+def test_reference_macro_dropin_compiles():
+    """A translation unit written against the reference call site,
+    cosmaSgemm(A, lda, B, ldb, C, ldc) with config.h macros, compiles and
+    links against the C ABI (host compiler only)."""
+    import shutil
+    import tempfile
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ missing")
+    src = r'''
+#define M 128
+#define N 64
+#define K 32
+#define ALPHA 1
+#define BETA 0.5
+#define COSMA_REFERENCE_MACROS
+#include "cosma/cosma.h"
+int call(const float* A, const float* B, float* Cp) { return cosmaSgemm(A, K, B, N, Cp, N); }
+'''
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "t.c")
+        open(p, "w").write(src)
+        subprocess.run(["gcc", "-c", "-I", os.path.join(ROOT, "include"), p, "-o",
+                        os.path.join(td, "t.o")], check=True)

@@ -1,0 +1,236 @@
+"""GPU parity of the HIP GEMM kernels against the CPU oracle.
+
+Bars:
+  * small-integer operands: every mode / layout / tile / split / edge case is
+    bitwise equal to the exact integer product (sums stay below 2^24);
+  * FP32 and BF16x3 on random data: componentwise |C - C*| <= c*K*u*(|A||B|)
+    with u = 2^-24 (FP32 accumulation) plus, for BF16x3, 2^-15 |A||B| for the
+    split; BF16x3 is also held to the exact three-product oracle;
+  * config 1 (1024^3, alpha 1, beta 0.5): normwise error vs the FP64 oracle
+    <= 1e-6, and agreement with the reference kernel's own arithmetic order
+    (oracle.refsim_sgemm) within 2*K*u*|A||B|;
+  * FP64: componentwise within 4*K*2^-53*(|A||B|) of the long-double oracle;
+  * split-K: bitwise repeatable.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def cc(dev):
+    import cucosma_amd
+
+    return cucosma_amd
+
+
+def _t(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _int_operands(rng, M, N, K, dt, lo=-8, hi=9):
+    A = rng.integers(lo, hi, (M, K)).astype(dt)
+    B = rng.integers(lo, hi, (K, N)).astype(dt)
+    C0 = rng.integers(lo, hi, (M, N)).astype(dt)
+    return A, B, C0
+
+
+MODES = [("fp32", np.float32), ("fp64", np.float64), ("bf16x3", np.float32)]
+LAYOUTS = [(False, False), (False, True), (True, False)]
+
+
+@pytest.mark.parametrize("mode,dt", MODES)
+@pytest.mark.parametrize("ta,tb", LAYOUTS)
+def test_integer_exact_all_tiles(cc, dev, mode, dt, ta, tb):
+    """Exact integer products: checks MFMA lane layouts, loaders (both
+    layouts), edges, split-K slices and the epilogue for every tile config."""
+    rng = np.random.default_rng(100)
+    ncfg = len(cc.tile_configs(cc._lib.MODES[mode]))
+    for (M, N, K) in ((256, 256, 64), (200, 136, 52)):
+        A, B, C0 = _int_operands(rng, M, N, K, dt)
+        want = (A.astype(np.int64) @ B.astype(np.int64)) * 2 - C0.astype(np.int64)
+        Ad = _t(A.T if ta else A, dev)
+        Bd = _t(B.T if tb else B, dev)
+        for cfg in range(ncfg):
+            for split in (1, 3):
+                Cd = _t(C0, dev)
+                cc.gemm(Ad, Bd, Cd, alpha=2.0, beta=-1.0, mode=mode, trans_a=ta, trans_b=tb,
+                        tile_cfg=cfg, split_k=split)
+                got = Cd.cpu().numpy()
+                assert np.array_equal(got, want.astype(dt)), (M, N, K, cfg, split)
+
+
+@pytest.mark.parametrize("mode,dt", MODES)
+def test_padded_leading_dims_and_ragged_k(cc, dev, mode, dt):
+    """K not a multiple of the vector width, rows padded (ld > cols)."""
+    rng = np.random.default_rng(101)
+    M, N, K = 70, 45, 50
+    pad = 4 if dt == np.float32 else 2
+    A, B, C0 = _int_operands(rng, M, N, K, dt)
+    Abig = np.zeros((M, K + 2 + pad), dt)
+    Abig[:, :K] = A
+    Bbig = np.zeros((K, N + 3 + pad), dt)
+    Bbig[:, :N] = B
+    Ad = _t(Abig, dev)[:, :K]
+    Bd = _t(Bbig, dev)[:, :N]
+    assert Ad.stride(0) % pad == 0 and Bd.stride(0) % pad == 0
+    Cd = _t(C0, dev)
+    cc.gemm(Ad, Bd, Cd, alpha=1.0, beta=1.0, mode=mode)
+    want = A.astype(np.int64) @ B.astype(np.int64) + C0.astype(np.int64)
+    assert np.array_equal(Cd.cpu().numpy(), want.astype(dt))
+
+
+def _bound_check(got, ref, A, B, slack, u, extra=0.0):
+    absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    K = A.shape[1]
+    lim = (slack * K * u + extra) * absprod + 1e-30
+    err = np.abs(got.astype(np.float64) - ref)
+    assert np.all(err <= lim), float((err / lim).max())
+
+
+@pytest.mark.parametrize("ta,tb", LAYOUTS)
+@pytest.mark.parametrize("shape", [(64, 64, 64), (333, 129, 517), (1, 300, 2048), (257, 3, 4096)])
+def test_fp32_random_vs_f64(cc, dev, ta, tb, shape):
+    M, N, K = shape
+    A, B = oracle.workload_square_flat(M, N, K, seed=1)
+    ref = oracle.gemm_f64(A, B)
+    Cd = cc.gemm(_t(A.T if ta else A, dev), _t(B.T if tb else B, dev), mode="fp32",
+                 trans_a=ta, trans_b=tb)
+    _bound_check(Cd.cpu().numpy(), ref, A, B, 2.0, 2.0**-24)
+
+
+def test_bf16x3_vs_split_oracle_and_f64(cc, dev):
+    M, N, K = 300, 200, 1000
+    A, B = oracle.workload_square_flat(M, N, K, seed=1)
+    got = cc.gemm(_t(A, dev), _t(B, dev), mode="bf16x3").cpu().numpy()
+    exact3 = oracle.gemm_bf16x3(A, B)  # hi*hi + hi*lo + lo*hi accumulated in double
+    _bound_check(got, exact3, A, B, 2.0, 2.0**-24)
+    ref = oracle.gemm_f64(A, B)
+    _bound_check(got, ref, A, B, 2.0, 2.0**-24, extra=2.0**-15)
+
+
+def test_config1_plain_sgemm_parity(cc, dev):
+    """1024^3, alpha 1, beta 0.5, A/B/C uniform[-1,1] from one MT19937(0)
+    stream, through the compile-time specialisation and the fixed-shape
+    entry point."""
+    A, B, C0 = oracle.workload_plain_1024()
+    ref = oracle.gemm_f64(A, B, C0, 1.0, 0.5)
+    refsim = oracle.refsim_sgemm(A, B, C0, 1.0, 0.5)  # the reference kernel's arithmetic
+    Ad, Bd = _t(A, dev), _t(B, dev)
+    for fn in (cc.lib.cosma_sgemm_1024x1024x1024, cc.lib.cosma_sgemm):
+        Cd = _t(C0, dev)
+        torch.cuda.synchronize()
+        st = fn(1.0, Ad.data_ptr(), 1024, Bd.data_ptr(), 1024, 0.5, Cd.data_ptr(), 1024)
+        assert st == 0
+        torch.cuda.synchronize()
+        got = Cd.cpu().numpy()
+        nw = oracle.normwise(got, ref, A, B)
+        assert nw <= 1e-6, nw
+        absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+        diff = np.abs(got.astype(np.float64) - refsim)
+        assert np.all(diff <= 2 * 1024 * 2.0**-24 * absprod + 1e-30)
+        # and the reference's own harness check (max-abs vs a comparator, tol 0.1)
+        assert diff.max() < 0.1
+
+
+def test_splitk_deterministic_and_consistent(cc, dev):
+    M, N, K = 96, 160, 8192
+    A, B = oracle.workload_square_flat(M, N, K, seed=1)
+    Ad, Bd = _t(A, dev), _t(B, dev)
+    ref = oracle.gemm_f64(A, B)
+    for mode in ("fp32", "bf16x3"):
+        for split in (2, 7, 16):
+            first = cc.gemm(Ad, Bd, mode=mode, split_k=split, tile_cfg=1).cpu().numpy()
+            for _ in range(3):
+                again = cc.gemm(Ad, Bd, mode=mode, split_k=split, tile_cfg=1).cpu().numpy()
+                assert np.array_equal(first.view(np.uint32), again.view(np.uint32))
+            extra = 2.0**-15 if mode == "bf16x3" else 0.0
+            _bound_check(first, ref, A, B, 2.0, 2.0**-24, extra)
+
+
+def test_fp64_vs_long_double_with_outliers(cc, dev):
+    A, B, C0 = oracle.workload_dgemm(200, 150, 300, seed=2)
+    ref = oracle.gemm_ld(A, B, C0, 1.0, 1.0)
+    for split in (1, 4):
+        Cd = _t(C0, dev)
+        cc.gemm(_t(A, dev), _t(B, dev), Cd, alpha=1.0, beta=1.0, mode="fp64", split_k=split)
+        got = Cd.cpu().numpy()
+        absprod = np.abs(A) @ np.abs(B) + np.abs(C0)
+        assert np.all(np.abs(got - ref) <= 4 * 300 * 2.0**-53 * absprod)
+
+
+@pytest.mark.parametrize("mode,dt", MODES)
+def test_alpha_beta_semantics(cc, dev, mode, dt):
+    rng = np.random.default_rng(102)
+    A, B, C0 = _int_operands(rng, 40, 30, 20, dt)
+    Ad, Bd = _t(A, dev), _t(B, dev)
+    # beta == 0 never reads C: NaN in C must not leak
+    Cd = torch.full((40, 30), float("nan"), dtype=Ad.dtype, device=dev)
+    cc.gemm(Ad, Bd, Cd, alpha=1.0, beta=0.0, mode=mode)
+    assert np.array_equal(Cd.cpu().numpy(), (A.astype(np.int64) @ B.astype(np.int64)).astype(dt))
+    Cd = torch.full((40, 30), float("nan"), dtype=Ad.dtype, device=dev)
+    cc.gemm(Ad, Bd, Cd, alpha=1.0, beta=0.0, mode=mode, split_k=3)
+    assert np.isfinite(Cd.cpu().numpy()).all()
+    # alpha == 0: C = beta*C, A/B not read
+    Cd = _t(C0, dev)
+    cc.gemm(Ad, Bd, Cd, alpha=0.0, beta=0.5, mode=mode)
+    assert np.array_equal(Cd.cpu().numpy(), (C0 * 0.5).astype(dt))
+    # K == 0: C = beta*C
+    Cd = _t(C0, dev)
+    cc.gemm(Ad[:, :0].contiguous(), Bd[:0, :].contiguous(), Cd, alpha=1.0, beta=2.0, mode=mode)
+    assert np.array_equal(Cd.cpu().numpy(), (C0 * 2).astype(dt))
+
+
+def test_residual_stats_on_gpu_match_host(cc, dev):
+    rng = np.random.default_rng(103)
+    R = rng.normal(0, 1, (513, 257))
+    R[3, 4] = 0.0
+    Cm = (R + rng.normal(0, 1e-5, R.shape)).astype(np.float32)
+    st_gpu = cc.error_stats(_t(Cm, dev), _t(R, dev))
+    st_cpu = oracle.err_stats(Cm, R)
+    for key in ("sum_sq_err", "sum_sq_ref"):
+        assert np.isclose(st_gpu[key], st_cpu[key], rtol=1e-12)
+    for key in ("max_abs", "max_rel", "max_ref"):
+        assert st_gpu[key] == st_cpu[key]
+    # deterministic: bitwise repeatable
+    again = cc.error_stats(_t(Cm, dev), _t(R, dev))
+    assert again == st_gpu
+    X = rng.normal(0, 1, (100, 77)).astype(np.float32)
+    assert np.isclose(cc.sumsq(_t(X, dev)), oracle.sumsq(X), rtol=1e-12)
+    Y = cc.to_f64(_t(X, dev)).cpu().numpy()
+    assert np.array_equal(Y, X.astype(np.float64))
+
+
+def test_caller_workspace(cc, dev):
+    M, N, K = 64, 64, 4096
+    A, B = oracle.workload_square_flat(M, N, K, seed=1)
+    Ad, Bd = _t(A, dev), _t(B, dev)
+    need = cc.workspace_bytes(M, N, K, split_k=8, tile_cfg=1)
+    assert need == 8 * M * N * 4
+    ws = torch.empty(need // 4, dtype=torch.float32, device=dev)
+    c1 = cc.gemm(Ad, Bd, split_k=8, tile_cfg=1, workspace=ws).cpu().numpy()
+    small = torch.empty(16, dtype=torch.float32, device=dev)
+    with pytest.raises(cc.CosmaError):
+        cc.gemm(Ad, Bd, split_k=8, tile_cfg=1, workspace=small)
+    # installed workspace
+    assert cc.lib.cosma_workspace_set(C.c_void_p(ws.data_ptr()), need) == 0
+    c2 = cc.gemm(Ad, Bd, split_k=8, tile_cfg=1).cpu().numpy()
+    assert cc.lib.cosma_workspace_set(None, 0) == 0
+    assert np.array_equal(c1, c2)
+
+
+def test_fp64_device_reference_matches_host_oracle(cc, dev):
+    """The GPU FP64 product used for large-size error reports agrees with the
+    host FP64 oracle."""
+    A, B = oracle.workload_square_flat(256, 192, 3000, seed=1)
+    R = cc.fp64_product(_t(A, dev), _t(B, dev)).cpu().numpy()
+    ref = oracle.gemm_f64(A, B)
+    absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    assert np.all(np.abs(R - ref) <= 4 * 3000 * 2.0**-53 * absprod)
