@@ -28,4 +28,7 @@ from .gemm import (  # noqa: F401
     workspace_bytes,
 )
 
+from .grouped import GroupedGemm, grouped_gemm  # noqa: F401,E402
+from .autograd import CosmaMatmul, matmul  # noqa: F401,E402
+
 __version__ = lib.cosma_version().decode()

@@ -59,6 +59,24 @@ class GemmDesc(C.Structure):
     ]
 
 
+class GroupProblem(C.Structure):
+    """Mirror of struct cosma_group_problem."""
+
+    _fields_ = [
+        ("m", C.c_int),
+        ("n", C.c_int),
+        ("k", C.c_int),
+        ("a", C.c_void_p),
+        ("lda", C.c_int64),
+        ("b", C.c_void_p),
+        ("ldb", C.c_int64),
+        ("c", C.c_void_p),
+        ("ldc", C.c_int64),
+        ("alpha", C.c_double),
+        ("beta", C.c_double),
+    ]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -93,6 +111,14 @@ def _load():
         "cosma_sumsq_f32": (i, [i, i, vp, i64, f64p, vp]),
         "cosma_sumsq_f64": (i, [i, i, vp, i64, f64p, vp]),
         "cosma_convert_f32_f64": (i, [i, i, vp, i64, vp, i64, vp]),
+        "cosma_grouped_plan_create": (i, [i, i, C.POINTER(GroupProblem), C.POINTER(i), i,
+                                          C.POINTER(vp)]),
+        "cosma_grouped_plan_run": (i, [vp, vp]),
+        "cosma_grouped_plan_run_graph": (i, [vp, vp]),
+        "cosma_grouped_plan_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
+        "cosma_grouped_plan_bucket": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i),
+                                          C.POINTER(i)]),
+        "cosma_grouped_plan_destroy": (i, [vp]),
         "cosma_status_string": (C.c_char_p, [i]),
         "cosma_version": (C.c_char_p, []),
     }

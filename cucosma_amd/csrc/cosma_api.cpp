@@ -78,7 +78,6 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
 }
 
 size_t elem_size(int mode) { return mode == COSMA_MODE_FP64 ? 8 : 4; }
-int vec_elems(int mode) { return mode == COSMA_MODE_FP64 ? 2 : 4; }
 
 std::mutex g_mu;
 struct DevState {
@@ -165,9 +164,8 @@ int validate(const cosma_gemm_desc* d) {
   const int64_t b_cols = d->trans_b ? d->k : d->n;
   if (d->lda < (a_cols > 1 ? a_cols : 1) || d->ldb < (b_cols > 1 ? b_cols : 1))
     return COSMA_STATUS_INVALID_VALUE;
-  const int v = vec_elems(d->mode);
-  if (((uintptr_t)d->a & 15u) || ((uintptr_t)d->b & 15u) || (d->lda % v) || (d->ldb % v))
-    return COSMA_STATUS_MISALIGNED;
+  const uintptr_t es = elem_size(d->mode);
+  if (((uintptr_t)d->a % es) || ((uintptr_t)d->b % es)) return COSMA_STATUS_MISALIGNED;
   return 0;
 }
 
@@ -279,9 +277,200 @@ int finish_stats(hipError_t e, const double* dev_out, double* out5, hipStream_t 
   return (int)hipStreamSynchronize(s);
 }
 
+int group_cfg(int mode, int m, int n) {
+  // Problems whose output spans at least 4 tiles of 128x128 take the large
+  // tile; smaller ones the 64x64 tile, which keeps more workgroups busy.
+  (void)mode;
+  return (cdiv(m, 128) * cdiv(n, 128) >= 4) ? 0 : 1;
+}
+
+}  // namespace
+
+struct cosma_grouped_plan_s {
+  int mode;
+  struct Bucket {
+    int cfg;
+    int count;
+    int tiles;
+    int blocks;
+    void* table;  // device GroupEntry array
+  };
+  std::vector<Bucket> buckets;
+  double flops = 0.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename G>
+int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*>& probs,
+                 int blocks_per_cu, cosma_grouped_plan_s::Bucket* out) {
+  const TileDims td = kDims[mode][cfg];
+  std::vector<cosma::GroupEntry<G>> table;
+  table.reserve(probs.size());
+  int64_t tiles = 0;
+  for (const cosma_group_problem* q : probs) {
+    cosma::GroupEntry<G> e{};
+    e.A = static_cast<const G*>(q->a);
+    e.B = static_cast<const G*>(q->b);
+    e.C = static_cast<G*>(q->c);
+    e.lda = q->lda;
+    e.ldb = q->ldb;
+    e.ldc = q->ldc;
+    e.M = q->m;
+    e.N = q->n;
+    e.K = q->k;
+    e.tiles_n = cdiv(q->n, td.bn);
+    e.tile_start = (int)tiles;
+    e.alpha = q->alpha;
+    e.beta = q->beta;
+    tiles += (int64_t)cdiv(q->m, td.bm) * e.tiles_n;
+    if (tiles > 0x7fffffff) return COSMA_STATUS_INVALID_VALUE;
+    table.push_back(e);
+  }
+  out->cfg = cfg;
+  out->count = (int)table.size();
+  out->tiles = (int)tiles;
+  const int occ = blocks_per_cu > 0 ? blocks_per_cu : cosma::grouped_occupancy(mode, cfg);
+  int64_t blocks = (int64_t)occ * kCUs;
+  if (blocks > tiles) blocks = tiles;
+  out->blocks = (int)(blocks > 0 ? blocks : 1);
+  out->table = nullptr;
+  const size_t bytes = table.size() * sizeof(cosma::GroupEntry<G>);
+  hipError_t e = hipMalloc(&out->table, bytes);
+  if (e != hipSuccess) {
+    out->table = nullptr;
+    return (int)e;
+  }
+  e = hipMemcpy(out->table, table.data(), bytes, hipMemcpyHostToDevice);
+  return (int)e;
+}
+
+void free_plan(cosma_grouped_plan_s* p) {
+  if (!p) return;
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  for (auto& b : p->buckets)
+    if (b.table) hipFree(b.table);
+  delete p;
+}
+
+int launch_plan(cosma_grouped_plan_s* p, hipStream_t s) {
+  for (auto& b : p->buckets) {
+    if (b.tiles == 0) continue;
+    hipError_t e = cosma::run_grouped(p->mode, b.cfg, b.table, b.count, b.tiles, b.blocks, s);
+    if (e != hipSuccess) return (int)e;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* problems,
+                              const int* tile_cfgs, int blocks_per_cu, cosma_grouped_plan* plan) {
+  if (!plan) return COSMA_STATUS_INVALID_VALUE;
+  *plan = nullptr;
+  if (mode < 0 || mode > 2 || count < 0 || (count > 0 && !problems))
+    return COSMA_STATUS_INVALID_VALUE;
+  std::vector<std::vector<const cosma_group_problem*>> by_cfg(kNumCfg[mode]);
+  double flops = 0.0;
+  for (int i = 0; i < count; ++i) {
+    const cosma_group_problem& q = problems[i];
+    cosma_gemm_desc d = make_desc(mode, q.m, q.n, q.k, q.alpha, q.a, (int)0, q.b, 0, q.beta, q.c,
+                                  0, nullptr);
+    d.lda = q.lda;
+    d.ldb = q.ldb;
+    d.ldc = q.ldc;
+    int st = validate(&d);
+    if (st != 0) return st;
+    if (q.m == 0 || q.n == 0) continue;
+    if (q.k > 0 && q.alpha != 0.0 && ((uintptr_t)q.c == 0)) return COSMA_STATUS_INVALID_VALUE;
+    int cfg = (tile_cfgs && tile_cfgs[i] >= 0) ? tile_cfgs[i] : group_cfg(mode, q.m, q.n);
+    if (cfg >= kNumCfg[mode]) return COSMA_STATUS_INVALID_VALUE;
+    by_cfg[cfg].push_back(&q);
+    flops += 2.0 * q.m * q.n * (double)q.k;
+  }
+  cosma_grouped_plan_s* p = new cosma_grouped_plan_s();
+  p->mode = mode;
+  p->flops = flops;
+  for (int cfg = 0; cfg < kNumCfg[mode]; ++cfg) {
+    if (by_cfg[cfg].empty()) continue;
+    cosma_grouped_plan_s::Bucket b{};
+    int st = mode == COSMA_MODE_FP64 ? build_bucket<double>(mode, cfg, by_cfg[cfg], blocks_per_cu, &b)
+                                     : build_bucket<float>(mode, cfg, by_cfg[cfg], blocks_per_cu, &b);
+    p->buckets.push_back(b);
+    if (st != 0) {
+      free_plan(p);
+      return st;
+    }
+  }
+  *plan = p;
+  return 0;
+}
+
+int cosma_grouped_plan_run(cosma_grouped_plan plan, void* stream) {
+  if (!plan) return COSMA_STATUS_INVALID_VALUE;
+  return launch_plan(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_grouped_plan_run_graph(cosma_grouped_plan plan, void* stream) {
+  if (!plan) return COSMA_STATUS_INVALID_VALUE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!plan->exec) {
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return (int)e;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int st = (e == hipSuccess) ? launch_plan(plan, cs) : (int)e;
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    hipStreamDestroy(cs);
+    if (st != 0) {
+      if (g) hipGraphDestroy(g);
+      return st;
+    }
+    if (e2 != hipSuccess) return (int)e2;
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      hipGraphDestroy(g);
+      return (int)e;
+    }
+    plan->graph = g;
+    plan->exec = x;
+  }
+  return (int)hipGraphLaunch(plan->exec, s);
+}
+
+int cosma_grouped_plan_info(cosma_grouped_plan plan, int* buckets, int* total_tiles, double* flops) {
+  if (!plan) return COSMA_STATUS_INVALID_VALUE;
+  int tiles = 0;
+  for (auto& b : plan->buckets) tiles += b.tiles;
+  if (buckets) *buckets = (int)plan->buckets.size();
+  if (total_tiles) *total_tiles = tiles;
+  if (flops) *flops = plan->flops;
+  return 0;
+}
+
+int cosma_grouped_plan_bucket(cosma_grouped_plan plan, int b, int* tile_cfg, int* count, int* tiles,
+                              int* blocks) {
+  if (!plan || b < 0 || b >= (int)plan->buckets.size()) return COSMA_STATUS_INVALID_VALUE;
+  const auto& k = plan->buckets[b];
+  if (tile_cfg) *tile_cfg = k.cfg;
+  if (count) *count = k.count;
+  if (tiles) *tiles = k.tiles;
+  if (blocks) *blocks = k.blocks;
+  return 0;
+}
+
+int cosma_grouped_plan_destroy(cosma_grouped_plan plan) {
+  free_plan(plan);
+  return 0;
+}
+
 
 void cosma_gemm_desc_init(cosma_gemm_desc* d) {
   if (!d) return;
@@ -465,7 +654,7 @@ const char* cosma_status_string(int status) {
   switch (status) {
     case COSMA_STATUS_SUCCESS: return "success";
     case COSMA_STATUS_INVALID_VALUE: return "invalid value (size, pointer, leading dimension, split or config)";
-    case COSMA_STATUS_MISALIGNED: return "operand breaks the 16-byte row alignment contract";
+    case COSMA_STATUS_MISALIGNED: return "operand base not aligned to its element size";
     case COSMA_STATUS_NOT_SUPPORTED: return "combination not supported";
     case COSMA_STATUS_WORKSPACE: return "split-K workspace unavailable while the stream is capturing";
     default: return status > 0 ? hipGetErrorString((hipError_t)status) : "unknown status";

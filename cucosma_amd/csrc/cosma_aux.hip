@@ -141,6 +141,33 @@ extern const int kF64NumTiles;
 extern const TileInfo kBF16x3Tiles[];
 extern const int kBF16x3NumTiles;
 
+hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int blocks, hipStream_t s);
+hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int blocks, hipStream_t s);
+hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, int blocks,
+                              hipStream_t s);
+int grouped_occupancy_f32(int cfg);
+int grouped_occupancy_f64(int cfg);
+int grouped_occupancy_bf16x3(int cfg);
+
+hipError_t run_grouped(int mode, int cfg, const void* table, int count, int total_tiles, int blocks,
+                       hipStream_t s) {
+  switch (mode) {
+    case kF32: return run_grouped_f32(cfg, table, count, total_tiles, blocks, s);
+    case kF64: return run_grouped_f64(cfg, table, count, total_tiles, blocks, s);
+    case kBF16x3: return run_grouped_bf16x3(cfg, table, count, total_tiles, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int grouped_occupancy(int mode, int cfg) {
+  switch (mode) {
+    case kF32: return grouped_occupancy_f32(cfg);
+    case kF64: return grouped_occupancy_f64(cfg);
+    case kBF16x3: return grouped_occupancy_bf16x3(cfg);
+    default: return 1;
+  }
+}
+
 int num_tile_cfgs(int mode) {
   switch (mode) {
     case kF32: return kF32NumTiles;

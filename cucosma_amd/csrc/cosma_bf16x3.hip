@@ -23,4 +23,23 @@ hipError_t run_gemm_bf16x3(const Problem& p) {
   }
 }
 
+hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, int blocks,
+                         hipStream_t s) {
+  switch (cfg) {
+    case 0: return B3C0::grouped(table, count, total, blocks, s);
+    case 1: return B3C1::grouped(table, count, total, blocks, s);
+    case 2: return B3C2::grouped(table, count, total, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int grouped_occupancy_bf16x3(int cfg) {
+  switch (cfg) {
+    case 0: return B3C0::grouped_occupancy();
+    case 1: return B3C1::grouped_occupancy();
+    case 2: return B3C2::grouped_occupancy();
+    default: return 1;
+  }
+}
+
 }  // namespace cosma

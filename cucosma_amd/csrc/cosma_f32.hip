@@ -25,4 +25,25 @@ hipError_t run_gemm_f32(const Problem& p) {
   }
 }
 
+hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int blocks,
+                         hipStream_t s) {
+  switch (cfg) {
+    case 0: return F32C0::grouped(table, count, total, blocks, s);
+    case 1: return F32C1::grouped(table, count, total, blocks, s);
+    case 2: return F32C2::grouped(table, count, total, blocks, s);
+    case 3: return F32C3::grouped(table, count, total, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int grouped_occupancy_f32(int cfg) {
+  switch (cfg) {
+    case 0: return F32C0::grouped_occupancy();
+    case 1: return F32C1::grouped_occupancy();
+    case 2: return F32C2::grouped_occupancy();
+    case 3: return F32C3::grouped_occupancy();
+    default: return 1;
+  }
+}
+
 }  // namespace cosma

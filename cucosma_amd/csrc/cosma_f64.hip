@@ -22,4 +22,23 @@ hipError_t run_gemm_f64(const Problem& p) {
   }
 }
 
+hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int blocks,
+                         hipStream_t s) {
+  switch (cfg) {
+    case 0: return F64C0::grouped(table, count, total, blocks, s);
+    case 1: return F64C1::grouped(table, count, total, blocks, s);
+    case 2: return F64C2::grouped(table, count, total, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int grouped_occupancy_f64(int cfg) {
+  switch (cfg) {
+    case 0: return F64C0::grouped_occupancy();
+    case 1: return F64C1::grouped_occupancy();
+    case 2: return F64C2::grouped_occupancy();
+    default: return 1;
+  }
+}
+
 }  // namespace cosma

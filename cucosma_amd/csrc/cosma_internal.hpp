@@ -37,6 +37,22 @@ struct TileInfo {
   int lds_bytes;
 };
 
+// One problem of a grouped launch, as the grouped kernel reads it from
+// device memory.  tile_start: index of the problem's first output tile in
+// the launch's tile numbering (ascending over the table).
+template <typename G>
+struct GroupEntry {
+  const G* A;
+  const G* B;
+  G* C;
+  int64_t lda, ldb, ldc;
+  int M, N, K;
+  int tiles_n;
+  int tile_start;
+  int pad_;
+  double alpha, beta;
+};
+
 int num_tile_cfgs(int mode);
 const TileInfo* tile_info(int mode, int cfg);  // nullptr when out of range
 
@@ -44,6 +60,13 @@ const TileInfo* tile_info(int mode, int cfg);  // nullptr when out of range
 hipError_t run_gemm_f32(const Problem& p);
 hipError_t run_gemm_f64(const Problem& p);
 hipError_t run_gemm_bf16x3(const Problem& p);
+
+// Grouped persistent launch of `blocks` workgroups over `count` table
+// entries (device memory) holding `total_tiles` tiles of tile config cfg.
+hipError_t run_grouped(int mode, int cfg, const void* table, int count, int total_tiles, int blocks,
+                       hipStream_t s);
+// Workgroups of the grouped kernel of (mode, cfg) that fit on one CU.
+int grouped_occupancy(int mode, int cfg);
 
 // C = beta*C over the M x N window (alpha == 0 or K == 0).
 hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, hipStream_t s);

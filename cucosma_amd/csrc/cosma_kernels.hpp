@@ -28,13 +28,16 @@
 //
 // MFMA operand mapping (16x16 output sub-tile, 64 lanes, g = lane/16,
 // r16 = lane%16): lane supplies A(row r16, k-group g) and B(k-group g, col
-// r16); the accumulator holds C(4g + r, r16) in element r.  Each lane reads
+// r16); for FP32/BF16 products the accumulator holds C(4g + r, r16) in
+// element r, for FP64 C(g + 4r, r16) (acc_row()).  Each lane reads
 // KPL consecutive k of its row with one 16-byte LDS read, and step s of the
 // MFMA sequence consumes element s of that read for both operands, so A and B
 // always agree on which k an MFMA slot carries.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "cosma_internal.hpp"
 
 namespace cosma {
 
@@ -108,6 +111,8 @@ struct GemmParams {
   int split;   // number of K slices
   int kslice;  // K per slice, a multiple of the tile's BK
   int group_m; // tile rows per group in the grouped tile order
+  int vec_a;   // 1: A's rows start on 16-byte boundaries (vector loads allowed)
+  int vec_b;   // 1: same for B
   E alpha, beta;
 };
 
@@ -171,8 +176,11 @@ struct TileLoader {
 
   GV v[NREG];
 
-  __device__ __forceinline__ static GV load_vec(const G* p, bool row_ok, int e_valid) {
-    if (!EDGE || (row_ok && e_valid >= GVEC)) return *reinterpret_cast<const GV*>(p);
+  // A whole 16-byte vector when it is in range and aligned; otherwise guarded
+  // element loads (EDGE only: the unchecked instantiation is launched only
+  // for aligned, tile-divisible problems).
+  __device__ __forceinline__ static GV load_vec(const G* p, bool row_ok, int e_valid, bool vec_ok) {
+    if (!EDGE || (vec_ok && row_ok && e_valid >= GVEC)) return *reinterpret_cast<const GV*>(p);
     GV out;
 #pragma unroll
     for (int e = 0; e < GVEC; ++e) out[e] = (row_ok && e < e_valid) ? p[e] : G(0);
@@ -183,7 +191,7 @@ struct TileLoader {
   // rmax: rows in the problem (M or N); k0: first k of the chunk; kend: end of
   // the valid k range of this split.
   __device__ __forceinline__ void load(const G* __restrict__ base, int64_t ld, int r0, int rmax,
-                                       int k0, int kend, int tid) {
+                                       int k0, int kend, int tid, bool vec_ok) {
     if constexpr (KMAJ) {
       constexpr int VPR = BK / GVEC;  // vectors per row
 #pragma unroll
@@ -195,7 +203,7 @@ struct TileLoader {
         const int gr = r0 + r;
         const int gk = k0 + kv * GVEC;
         const G* p = base + (int64_t)gr * ld + gk;
-        v[i] = load_vec(p, gr < rmax, kend - gk);
+        v[i] = load_vec(p, gr < rmax, kend - gk, vec_ok);
       }
     } else {
       constexpr int RB = R / GVEC;
@@ -210,7 +218,7 @@ struct TileLoader {
         for (int u = 0; u < GVEC; ++u) {
           const int gk = k0 + kb * GVEC + u;
           const G* p = base + (int64_t)gk * ld + gr;
-          v[i * GVEC + u] = load_vec(p, gk < kend, rmax - gr);
+          v[i * GVEC + u] = load_vec(p, gk < kend, rmax - gr, vec_ok);
         }
       }
     }
@@ -271,17 +279,25 @@ __device__ __forceinline__ typename Traits<MODE>::Acc zero_acc() {
   return a;
 }
 
-// Epilogue of one 16x16 sub-tile element set: split == 1 writes
-// alpha*acc (+ beta*C when beta != 0; C is not read when beta == 0);
-// split > 1 writes the raw partial to ws[z][row][col].
+// Row of accumulator element r held by lane group g of a 16x16 sub-tile.
+// v_mfma_f32_16x16x4_f32 and v_mfma_f32_16x16x32_bf16 give lane group g rows
+// 4g..4g+3; v_mfma_f64_16x16x4_f64 gives it rows g, g+4, g+8, g+12.
+template <int MODE>
+__device__ __forceinline__ int acc_row(int g, int r) {
+  return MODE == kF64 ? g + 4 * r : 4 * g + r;
+}
+
+// Epilogue of one 16x16 sub-tile (rows sub_row0.., column col of this lane):
+// split == 1 writes alpha*acc (+ beta*C when beta != 0; C is not read when
+// beta == 0); split > 1 writes the raw partial to ws[z][row][col].
 template <int MODE, bool EDGE>
-__device__ __forceinline__ void epilogue_store(const GemmParams<MODE>& p, int z, int row0, int col,
-                                               const typename Traits<MODE>::Acc& acc) {
+__device__ __forceinline__ void epilogue_store(const GemmParams<MODE>& p, int z, int sub_row0, int g,
+                                               int col, const typename Traits<MODE>::Acc& acc) {
   using E = typename Traits<MODE>::E;
   if (EDGE && col >= p.N) return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = row0 + r;
+    const int row = sub_row0 + acc_row<MODE>(g, r);
     if (EDGE && row >= p.M) continue;
     if (p.split == 1) {
       E v = p.alpha * acc[r];
@@ -373,17 +389,18 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
   }
 }
 
-// The GEMM kernel.  Grid: tiles_m * tiles_n * split workgroups (1-D).
+// One output tile (tm, tn) of split z: the K-slice main loop and the
+// epilogue.  lds holds two stages.  Ends with every wave past a barrier, so
+// a caller may start another tile on the same LDS right away.
 // EDGE == false requires M % BM == 0, N % BN == 0 and K % BK == 0.
 template <int MODE, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, bool EDGE>
-__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
-    gemm_kernel(GemmParams<MODE> p) {
+__device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int tn, int z,
+                                          typename Traits<MODE>::S* lds) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   using T = Traits<MODE>;
   using S = typename T::S;
   constexpr int NT = Sh::NT;
   constexpr int LDK = Sh::LDK;
-  __shared__ __attribute__((aligned(16))) S lds[2 * Sh::STAGE];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -391,8 +408,6 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   const int wm0 = (wave / Sh::WAVES_N) * WM;
   const int wn0 = (wave % Sh::WAVES_N) * WN;
 
-  int tm, tn, z;
-  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
   const int m0 = tm * BM;
   const int n0 = tn * BN;
   const int kbeg = z * p.kslice;
@@ -416,8 +431,8 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   };
 
   if (nk > 0) {
-    la.load(p.A, p.lda, m0, p.M, kbeg, kend, tid);
-    lb.load(p.B, p.ldb, n0, p.N, kbeg, kend, tid);
+    la.load(p.A, p.lda, m0, p.M, kbeg, kend, tid, p.vec_a);
+    lb.load(p.B, p.ldb, n0, p.N, kbeg, kend, tid, p.vec_b);
     store_stage(0);
     __syncthreads();
   }
@@ -426,8 +441,8 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     const bool more = it + 1 < nk;
     if (more) {
       const int k0 = kbeg + (it + 1) * BK;
-      la.load(p.A, p.lda, m0, p.M, k0, kend, tid);
-      lb.load(p.B, p.ldb, n0, p.N, k0, kend, tid);
+      la.load(p.A, p.lda, m0, p.M, k0, kend, tid, p.vec_a);
+      lb.load(p.B, p.ldb, n0, p.N, k0, kend, tid, p.vec_b);
     }
     compute_stage<MODE, BM, BN, BK, WM, WN>(stage_ptr(cur), acc, wm0, wn0, lane);
     if (more) store_stage(cur ^ 1);
@@ -440,8 +455,68 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   for (int i = 0; i < Sh::TM; ++i)
 #pragma unroll
     for (int j = 0; j < Sh::TN; ++j)
-      epilogue_store<MODE, EDGE>(p, z, m0 + wm0 + i * 16 + g * 4, n0 + wn0 + j * 16 + r16,
-                                 acc[i][j]);
+      epilogue_store<MODE, EDGE>(p, z, m0 + wm0 + i * 16, g, n0 + wn0 + j * 16 + r16, acc[i][j]);
+}
+
+// The GEMM kernel.  Grid: tiles_m * tiles_n * split workgroups (1-D).
+template <int MODE, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, bool EDGE>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
+    gemm_kernel(GemmParams<MODE> p) {
+  using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+  __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
+  int tm, tn, z;
+  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
+  gemm_tile<MODE, BM, BN, BK, WM, WN, TA, TB, EDGE>(p, tm, tn, z, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped GEMM: many independent row-major problems in one persistent launch.
+// The table lists every problem with the index of its first tile in a global
+// tile numbering (tile_start, ascending).  Workgroup w processes tiles
+// w, w + gridDim.x, ... ; each tile is located by binary search in the table
+// and computed with the same gemm_tile as the single-problem kernel (split 1,
+// bounds-checked edges).  Tiles are numbered problem by problem, row-major
+// within a problem, so concurrently running workgroups share operand panels.
+// ---------------------------------------------------------------------------
+template <int MODE, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
+    grouped_kernel(const GroupEntry<typename Traits<MODE>::G>* __restrict__ table, int count,
+                   int total_tiles) {
+  using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+  using E = typename Traits<MODE>::E;
+  __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
+  for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    int lo = 0, hi = count - 1;  // last entry with tile_start <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (table[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+    }
+    const GroupEntry<typename Traits<MODE>::G>& e = table[lo];
+    GemmParams<MODE> p;
+    p.A = e.A;
+    p.B = e.B;
+    p.C = e.C;
+    p.ws = nullptr;
+    p.lda = e.lda;
+    p.ldb = e.ldb;
+    p.ldc = e.ldc;
+    p.M = e.M;
+    p.N = e.N;
+    p.K = e.K;
+    p.tiles_m = (e.M + BM - 1) / BM;
+    p.tiles_n = e.tiles_n;
+    p.split = 1;
+    p.kslice = ((e.K + BK - 1) / BK) * BK;
+    p.group_m = 1;
+    p.vec_a = ((uintptr_t)e.A % 16 == 0) && (e.lda % Traits<MODE>::GVEC == 0);
+    p.vec_b = ((uintptr_t)e.B % 16 == 0) && (e.ldb % Traits<MODE>::GVEC == 0);
+    p.alpha = (E)e.alpha;
+    p.beta = (E)e.beta;
+    const int local = t - e.tile_start;
+    const int tm = local / e.tiles_n;
+    const int tn = local - tm * e.tiles_n;
+    gemm_tile<MODE, BM, BN, BK, WM, WN, false, false, true>(p, tm, tn, 0, lds);
+  }
 }
 
 }  // namespace cosma

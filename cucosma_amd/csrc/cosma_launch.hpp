@@ -21,6 +21,22 @@ struct Tile {
     return hipGetLastError();
   }
 
+  static hipError_t grouped(const void* table, int count, int total, int blocks, hipStream_t s) {
+    using G = typename Traits<MODE>::G;
+    hipLaunchKernelGGL((grouped_kernel<MODE, BM, BN, BK, WM, WN>), dim3(blocks), dim3(Sh::NT), 0, s,
+                       static_cast<const GroupEntry<G>*>(table), count, total);
+    return hipGetLastError();
+  }
+
+  static int grouped_occupancy() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(&grouped_kernel<MODE, BM, BN, BK, WM, WN>), Sh::NT,
+            0) != hipSuccess)
+      return 1;
+    return n > 0 ? n : 1;
+  }
+
   static hipError_t run(const Problem& pr) {
     using G = typename Traits<MODE>::G;
     using E = typename Traits<MODE>::E;
@@ -46,7 +62,11 @@ struct Tile {
     const int64_t nblk64 = (int64_t)gp.tiles_m * gp.tiles_n * pr.split;
     if (nblk64 <= 0 || nblk64 > 0x7fffffff) return hipErrorInvalidValue;
     const int nblk = (int)nblk64;
-    const bool edge = (pr.M % BM) || (pr.N % BN) || (pr.K % BK);
+    constexpr int GVEC = Traits<MODE>::GVEC;
+    gp.vec_a = ((uintptr_t)pr.A % 16 == 0) && (pr.lda % GVEC == 0);
+    gp.vec_b = ((uintptr_t)pr.B % 16 == 0) && (pr.ldb % GVEC == 0);
+    // Unaligned rows take the bounds-checked instantiation with element loads.
+    const bool edge = (pr.M % BM) || (pr.N % BN) || (pr.K % BK) || !gp.vec_a || !gp.vec_b;
     hipError_t e;
     if (pr.ta == 0 && pr.tb == 0)
       e = edge ? go<false, false, true>(gp, nblk, pr.stream) : go<false, false, false>(gp, nblk, pr.stream);

@@ -28,11 +28,12 @@
  * Status codes: 0 success; negative = contract violation detected on the host
  * before any GPU work; positive = the hipError_t reported by the HIP runtime.
  *
- * Operand contract: every operand base pointer is 16-byte aligned and every
- * leading dimension of A and B is a multiple of 4 elements (FP32) or 2
- * elements (FP64), so each row starts on a 16-byte boundary -- the implicit
- * float4 contract of the reference's loaders (cuCOSMAV100.cuh:143-144,
- * 315-317), checked here.  C has no alignment requirement.
+ * Operand contract: A, B and C are aligned to their element size.  When A's
+ * and B's rows start on 16-byte boundaries (base 16-byte aligned, lda/ldb a
+ * multiple of 4 floats or 2 doubles -- the reference's implicit float4
+ * contract, cuCOSMAV100.cuh:143-144, 315-317) the kernels use 16-byte vector
+ * loads; any other leading dimension is accepted and served by the
+ * bounds-checked instantiation with element loads.
  */
 #ifndef COSMA_COSMA_H_
 #define COSMA_COSMA_H_
@@ -46,7 +47,7 @@ extern "C" {
 
 #define COSMA_STATUS_SUCCESS 0
 #define COSMA_STATUS_INVALID_VALUE (-1) /* negative size, null pointer, ld too small, bad split/config */
-#define COSMA_STATUS_MISALIGNED (-2)    /* base not 16-byte aligned or ld breaks 16-byte rows */
+#define COSMA_STATUS_MISALIGNED (-2)    /* operand base not aligned to its element size */
 #define COSMA_STATUS_NOT_SUPPORTED (-3) /* combination not built (e.g. both operands transposed) */
 #define COSMA_STATUS_WORKSPACE (-4)     /* split-K workspace missing while the stream is capturing */
 
@@ -156,6 +157,43 @@ int cosma_sumsq_f64(int rows, int cols, const double *X, int64_t ldx, double *ou
 /* Y = (double)X, element-wise over a rows x cols window. */
 int cosma_convert_f32_f64(int rows, int cols, const float *X, int64_t ldx, double *Y, int64_t ldy,
                           void *stream);
+
+/* ---- Grouped GEMM: many independent row-major problems, one persistent
+ * launch per tile-configuration bucket, replayable as one hipGraph.  No
+ * reference counterpart: the reference builds one binary per shape
+ * (Kernel/cuda/run.sh:185-313). ---- */
+typedef struct cosma_group_problem {
+    int m, n, k;
+    const void *a;
+    int64_t lda;
+    const void *b;
+    int64_t ldb;
+    void *c;
+    int64_t ldc;
+    double alpha, beta;
+} cosma_group_problem;
+
+typedef struct cosma_grouped_plan_s *cosma_grouped_plan;
+
+/* Validate the problems (same contract as cosma_gemm_ex, row-major only),
+ * bucket them by tile configuration (tile_cfgs[i] for problem i, or the
+ * library heuristic when tile_cfgs is NULL or tile_cfgs[i] < 0), and upload
+ * one device table per bucket.  Allocates and synchronises; do not call
+ * while capturing.  blocks_per_cu <= 0 uses the kernel's occupancy. */
+int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem *problems,
+                              const int *tile_cfgs, int blocks_per_cu, cosma_grouped_plan *plan);
+/* One persistent-kernel launch per bucket on stream (no allocation, no copy:
+ * capturable). */
+int cosma_grouped_plan_run(cosma_grouped_plan plan, void *stream);
+/* Launch the plan as a hipGraph: captured once on first use, then
+ * hipGraphLaunch on stream. */
+int cosma_grouped_plan_run_graph(cosma_grouped_plan plan, void *stream);
+/* buckets, total tiles, sum of 2*M*N*K over the problems. */
+int cosma_grouped_plan_info(cosma_grouped_plan plan, int *buckets, int *total_tiles, double *flops);
+/* Bucket b: tile config, problem count, tiles, persistent workgroups. */
+int cosma_grouped_plan_bucket(cosma_grouped_plan plan, int b, int *tile_cfg, int *count,
+                              int *tiles, int *blocks);
+int cosma_grouped_plan_destroy(cosma_grouped_plan plan);
 
 /* Human-readable text for a status code (static storage). */
 const char *cosma_status_string(int status);

@@ -88,9 +88,9 @@ def _desc(**kw):
         (dict(trans_a=2), -1),
         (dict(split_k=5000), -1),
         (dict(tile_cfg=99), -1),
-        (dict(a=0x1004), -2),
-        (dict(lda=66), -2),
-        (dict(mode=1, lda=65), -2),
+        (dict(a=0x1002), -2),
+        (dict(b=0x2001), -2),
+        (dict(mode=1, a=0x1004), -2),
         (dict(trans_a=1, trans_b=1), -3),
     ],
 )
@@ -116,7 +116,7 @@ def test_empty_problems_are_noops_on_host():
     assert lib.cosma_sgemm_bf16x3_mnk(4, 4, 4, 1.0, 0x1001, 4, 0x1000, 4, 0.0, 0x1000, 4, None) == -2
     assert lib.cosma_dgemm_mnk(4, 4, 4, 1.0, 0x1000, 3, 0x1000, 4, 0.0, 0x1000, 4, None) == -1
     assert lib.cosma_sgemm_8192x8192x8192(1.0, 0x1000, 100, 0x1000, 8192, 0.0, 0x1000, 8192) == -1
-    assert lib.cosma_dgemm_32768x1024x1024(1.0, 0x1008, 1024, 0x1000, 1024, 0.0, 0x1000, 1024) == -2
+    assert lib.cosma_dgemm_32768x1024x1024(1.0, 0x1004, 1024, 0x1000, 1024, 0.0, 0x1000, 1024) == -2
     m, n, k = C.c_int(), C.c_int(), C.c_int()
     lib.cosma_fixed_shape(C.byref(m), C.byref(n), C.byref(k))
     assert (m.value, n.value, k.value) == (1024, 1024, 1024)
