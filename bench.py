@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: SGEMM 8192 x 8192 x 8192, BF16x3 split precision vs plain FP32
-MFMA, with normwise error against an FP64 reference.
+MFMA, with normwise error against an FP64 reference (the four-product BF16x4
+variant is timed and checked beside them).
 
 One step = one C = A*B over the resident 8192^3 operands (A, B uniform[-1,1]
 from MT19937 seed 1, beta = 0).  `value` is the BF16x3 GEMM's throughput
@@ -63,14 +64,15 @@ def cpu_baseline(A: np.ndarray, B: np.ndarray, target_s: float = 12.0) -> dict:
     import oracle  # the CPU restatement; used here only as the timed baseline
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    rows = 2
+    rows = threads  # one row per thread per round; grow until the sample takes ~target_s
     while True:
         t0 = time.perf_counter()
         oracle.cpu_naive_sgemm(A, B, rows=rows, row0=0, nthreads=threads)
         dt = time.perf_counter() - t0
-        if dt >= target_s / 4 or rows >= 1024:
+        if dt >= target_s / 3 or rows >= 1024:
             break
-        rows = min(1024, max(rows * 2, int(rows * target_s / 4 / max(dt, 1e-3))))
+        grow = max(2, int(target_s / max(dt, 1e-3)))
+        rows = min(1024, rows * grow // threads * threads)
     flops = 2.0 * rows * N * K
     t0 = time.perf_counter()
     oracle.cpu_naive_sgemm(A, B, rows=1, row0=rows, nthreads=1)
@@ -94,6 +96,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error reports")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -128,6 +132,20 @@ def main() -> int:
         if st:
             raise cc.CosmaError(st, "fp32")
 
+    C4 = torch.empty((M, N), dtype=torch.float32, device=dev)
+
+    def run_bf16x4():
+        cc.gemm(Ad, Bd, C4, mode="bf16x4", stream=stream)
+
+    if args.copy_calibration:
+        # A plain device copy of known size (1 GiB read + 1 GiB write) in the
+        # same process, so counter runs can scale FETCH_SIZE / WRITE_SIZE.
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # Both fixed-shape entry points launch on the default (null) stream, which
     # is torch's current stream here.
     flops = 2.0 * M * N * K
@@ -139,9 +157,23 @@ def main() -> int:
     log(f"bf16x3: {ev_b:.3f} ms/launch ({flops / ev_b / 1e9:.1f} TFLOP/s) in {time.perf_counter() - t0:.1f} s")
     wall_p, ev_p, min_p = time_launches(run_fp32, stream, args.steps, args.warmup, torch)
     log(f"fp32:   {ev_p:.3f} ms/launch ({flops / ev_p / 1e9:.1f} TFLOP/s)")
+    wall_4, ev_4, min_4 = time_launches(run_bf16x4, stream, args.steps, args.warmup, torch)
+    log(f"bf16x4: {ev_4:.3f} ms/launch ({flops / ev_4 / 1e9:.1f} TFLOP/s)")
 
+    cfg_b, split_b = cc.heuristic(cc.MODE_BF16X3, M, N, K)
+    tile_b = cc.tile_configs(cc.MODE_BF16X3)[cfg_b]
+    kname = (f"gemm_kernel<BF16x3, {tile_b['BM']}x{tile_b['BN']}x{tile_b['BK']}, "
+             f"{tile_b['threads'] // 64} waves of {tile_b['WM']}x{tile_b['WN']}>")
     tf_b = flops / wall_b / 1e9
     tf_p = flops / wall_p / 1e9
+    tf_4 = flops / wall_4 / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_sgemm_8192.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tinfo = json.load(f)
+        if tinfo.get("kernel") == kname:
+            traffic = tinfo.get("bytes_per_launch")
     result = {
         "metric": METRIC,
         "value": round(tf_b, 3),
@@ -159,14 +191,17 @@ def main() -> int:
                                "bf16x3 vs plain fp32 mfma", "M": M, "N": N, "K": K},
         "bf16x3": {"tflops": round(tf_b, 3), "pct_useful_peak": round(100 * tf_b / PEAK_BF16X3, 2),
                    "pct_bf16_mfma_peak": round(100 * 3 * tf_b / PEAK_BF16, 2),
-                   "ms_per_launch_events": round(ev_b, 4)},
+                   "ms_per_launch_events": round(ev_b, 4), "kernel": kname},
         "plain_fp32": {"tflops": round(tf_p, 3), "pct_peak": round(100 * tf_p / PEAK_FP32, 2),
                        "ms_per_launch_events": round(ev_p, 4)},
+        "bf16x4": {"tflops": round(tf_4, 3), "pct_bf16_mfma_peak": round(100 * 4 * tf_4 / PEAK_BF16, 2),
+                   "ms_per_launch_events": round(ev_4, 4)},
         "speedup_bf16x3_vs_fp32": round(tf_b / tf_p, 3),
+        "speedup_bf16x4_vs_fp32": round(tf_4 / tf_p, 3),
         "roofline": {"bound": "mfma", "achieved": round(flops / ev_b / 1e9, 3),
                      "peak": round(PEAK_BF16X3, 1), "unit": "TFLOP/s",
-                     "frac": round(flops / ev_b / 1e9 / PEAK_BF16X3, 4), "traffic": None,
-                     "kernel": "gemm_kernel<BF16x3, 128x128x32>",
+                     "frac": round(flops / ev_b / 1e9 / PEAK_BF16X3, 4), "traffic": traffic,
+                     "kernel": kname,
                      "peak_basis": "MI355X dense bf16 MFMA 2516.6 TF / 3 products"},
     }
 
@@ -180,6 +215,9 @@ def main() -> int:
         spot = float(np.max(np.abs(dev_rows - ref_rows)) / np.max(np.abs(ref_rows)))
         eb = cc.normwise_error(Ad, Bd, Cb, R)
         ep = cc.normwise_error(Ad, Bd, Cp, R)
+        e4 = cc.normwise_error(Ad, Bd, C4, R)
+        result["bf16x4"]["normwise_err"] = e4["normwise"]
+        result["err_ratio_bf16x4_vs_fp32"] = e4["normwise"] / ep["normwise"] if ep["normwise"] else None
         result["bf16x3"]["normwise_err"] = eb["normwise"]
         result["bf16x3"]["max_rel_err"] = eb["max_rel"]
         result["plain_fp32"]["normwise_err"] = ep["normwise"]
@@ -187,7 +225,8 @@ def main() -> int:
         result["err_ratio_bf16x3_vs_fp32"] = eb["normwise"] / ep["normwise"] if ep["normwise"] else None
         result["fp64_reference_spotcheck_rel"] = spot
         del R
-        log(f"normwise: bf16x3 {eb['normwise']:.3e}  fp32 {ep['normwise']:.3e}")
+        log(f"normwise: bf16x3 {eb['normwise']:.3e}  fp32 {ep['normwise']:.3e}  "
+            f"bf16x4 {e4['normwise']:.3e}")
 
     if not args.no_cpu:
         log("cpu baseline (reference CPU verification loop on a row sample)")

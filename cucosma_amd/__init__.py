@@ -7,6 +7,7 @@ implementation.
 """
 from ._lib import (  # noqa: F401
     MODE_BF16X3,
+    MODE_BF16X4,
     MODE_FP32,
     MODE_FP64,
     CosmaError,
@@ -23,6 +24,7 @@ from .gemm import (  # noqa: F401
     fp64_product,
     sgemm,
     sgemm_bf16x3,
+    sgemm_bf16x4,
     sumsq,
     to_f64,
     workspace_bytes,

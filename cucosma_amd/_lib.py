@@ -20,7 +20,8 @@ STATUS_WORKSPACE = -4
 MODE_FP32 = 0
 MODE_FP64 = 1
 MODE_BF16X3 = 2
-MODES = {"fp32": MODE_FP32, "fp64": MODE_FP64, "bf16x3": MODE_BF16X3}
+MODE_BF16X4 = 3
+MODES = {"fp32": MODE_FP32, "fp64": MODE_FP64, "bf16x3": MODE_BF16X3, "bf16x4": MODE_BF16X4}
 
 
 class CosmaError(RuntimeError):

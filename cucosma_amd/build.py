@@ -16,7 +16,14 @@ CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcosma_amd.so")
 
-SOURCES = ["cosma_f32.hip", "cosma_f64.hip", "cosma_bf16x3.hip", "cosma_aux.hip", "cosma_api.cpp"]
+SOURCES = [
+    "cosma_f32.hip",
+    "cosma_f64.hip",
+    "cosma_bf16x3.hip",
+    "cosma_bf16x4.hip",
+    "cosma_aux.hip",
+    "cosma_api.cpp",
+]
 
 ARCH = "gfx950"
 # -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32.

@@ -38,12 +38,19 @@ constexpr int kCUs = 256;  // MI355X compute units
 struct TileDims {
   int bm, bn, bk;
 };
-constexpr TileDims kDims[3][4] = {
-    {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}},  // FP32
-    {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {0, 0, 0}},       // FP64
-    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {0, 0, 0}},      // BF16X3
+constexpr int kMaxCfg = 8;
+constexpr TileDims kDims[4][kMaxCfg] = {
+    // FP32
+    {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}},
+    // FP64
+    {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 16}, {64, 64, 32}},
+    // BF16X3
+    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64}},
+    // BF16X4 (same tiles as BF16X3)
+    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64}},
 };
-constexpr int kNumCfg[3] = {4, 3, 3};
+constexpr int kNumModes = 4;
+constexpr int kNumCfg[kNumModes] = {5, 5, 6, 6};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -53,22 +60,30 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 
 // Heuristic tile / split choice (replaced per shape by the autotuner's
 // measured choice when the caller passes tile_cfg / split_k).
-// Large outputs take the 128x128 tile; outputs too small to give every CU
-// work take smaller tiles and then split K so that about two workgroups per
-// CU exist, keeping every K slice at least 512 deep.
+// Each mode has a large tile measured fastest at 8192^3 / 32768x1024x1024
+// (FP32 128x128 with 4 waves, FP64 128x128 with 8 waves of 64x32, BF16x3
+// 256x256 with 8 waves of 128x64, BF16x4 256x128 with 8 waves of 64x64, which
+// fits its extra temporaries without spilling); it is used while it still gives every CU
+// two workgroups.  Smaller outputs step down to 128x64 and then 64x64 tiles
+// and split K, keeping every K
+// slice at least 512 deep.  Split-K aims at four workgroups per CU (the
+// measured optimum on 256x256x1048576).
 struct Choice {
   int cfg, split;
 };
+constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
+constexpr int large_cfg(int mode) { return mode == 0 ? 0 : (mode == 3 ? 2 : 3); }
+constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 constexpr Choice heuristic(int mode, int m, int n, int k) {
-  int cfg = 0;
-  if (tiles_of(mode, 0, m, n) < 2 * kCUs) {
-    cfg = (mode == 2) ? 1 : 2;  // 64x64 (bf16x3) or 128x64
-    if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;
+  int cfg = large_cfg(mode);
+  if (tiles_of(mode, cfg, m, n) < 2 * kCUs) {
+    cfg = medium_cfg(mode);
+    if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;  // 64x64
   }
   int tiles = tiles_of(mode, cfg, m, n);
   int split = 1;
   if (tiles < kCUs && k >= 1024) {
-    split = cdiv(2 * kCUs, tiles);
+    split = cdiv(4 * kCUs, tiles);
     int max_by_depth = k / 512;
     if (split > max_by_depth) split = max_by_depth;
     if (split > 64) split = 64;
@@ -147,7 +162,7 @@ int resolve_workspace(const cosma_gemm_desc* d, size_t need, hipStream_t s, void
 // Host-side contract check; returns 0, or a negative status.
 int validate(const cosma_gemm_desc* d) {
   if (!d) return COSMA_STATUS_INVALID_VALUE;
-  if (d->mode < 0 || d->mode > 2) return COSMA_STATUS_INVALID_VALUE;
+  if (d->mode < 0 || d->mode >= kNumModes) return COSMA_STATUS_INVALID_VALUE;
   if (d->m < 0 || d->n < 0 || d->k < 0) return COSMA_STATUS_INVALID_VALUE;
   if ((d->trans_a != 0 && d->trans_a != 1) || (d->trans_b != 0 && d->trans_b != 1))
     return COSMA_STATUS_INVALID_VALUE;
@@ -214,7 +229,8 @@ int run(const cosma_gemm_desc* d) {
   switch (d->mode) {
     case COSMA_MODE_FP32: e = cosma::run_gemm_f32(p); break;
     case COSMA_MODE_FP64: e = cosma::run_gemm_f64(p); break;
-    default: e = cosma::run_gemm_bf16x3(p); break;
+    case COSMA_MODE_BF16X3: e = cosma::run_gemm_bf16x3(p); break;
+    default: e = cosma::run_gemm_bf16x4(p); break;
   }
   return (int)e;
 }
@@ -373,7 +389,7 @@ int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* pr
                               const int* tile_cfgs, int blocks_per_cu, cosma_grouped_plan* plan) {
   if (!plan) return COSMA_STATUS_INVALID_VALUE;
   *plan = nullptr;
-  if (mode < 0 || mode > 2 || count < 0 || (count > 0 && !problems))
+  if (mode < 0 || mode >= kNumModes || count < 0 || (count > 0 && !problems))
     return COSMA_STATUS_INVALID_VALUE;
   std::vector<std::vector<const cosma_group_problem*>> by_cfg(kNumCfg[mode]);
   double flops = 0.0;
@@ -573,12 +589,12 @@ int cosma_sgemm_bf16x3_mnk(int m, int n, int k, float alpha, const float* A, int
 }
 
 int cosma_num_tile_configs(int mode) {
-  if (mode < 0 || mode > 2) return 0;
+  if (mode < 0 || mode >= kNumModes) return 0;
   return kNumCfg[mode];
 }
 
 int cosma_tile_config(int mode, int cfg, int* out7) {
-  if (!out7 || mode < 0 || mode > 2 || cfg < 0 || cfg >= kNumCfg[mode])
+  if (!out7 || mode < 0 || mode >= kNumModes || cfg < 0 || cfg >= kNumCfg[mode])
     return COSMA_STATUS_INVALID_VALUE;
   const cosma::TileInfo* t = cosma::tile_info(mode, cfg);
   if (!t || t->BM != kDims[mode][cfg].bm || t->BN != kDims[mode][cfg].bn ||
@@ -595,7 +611,7 @@ int cosma_tile_config(int mode, int cfg, int* out7) {
 }
 
 int cosma_heuristic(int mode, int m, int n, int k, int* tile_cfg, int* split_k) {
-  if (mode < 0 || mode > 2 || m < 0 || n < 0 || k < 0 || !tile_cfg || !split_k)
+  if (mode < 0 || mode >= kNumModes || m < 0 || n < 0 || k < 0 || !tile_cfg || !split_k)
     return COSMA_STATUS_INVALID_VALUE;
   Choice ch = heuristic(mode, m, n, k);
   *tile_cfg = ch.cfg;

@@ -140,6 +140,8 @@ extern const TileInfo kF64Tiles[];
 extern const int kF64NumTiles;
 extern const TileInfo kBF16x3Tiles[];
 extern const int kBF16x3NumTiles;
+extern const TileInfo kBF16x4Tiles[];
+extern const int kBF16x4NumTiles;
 
 hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int blocks, hipStream_t s);
 hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int blocks, hipStream_t s);
@@ -148,6 +150,9 @@ hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, 
 int grouped_occupancy_f32(int cfg);
 int grouped_occupancy_f64(int cfg);
 int grouped_occupancy_bf16x3(int cfg);
+hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, int total, int blocks,
+                              hipStream_t s);
+int grouped_occupancy_bf16x4(int cfg);
 
 hipError_t run_grouped(int mode, int cfg, const void* table, int count, int total_tiles, int blocks,
                        hipStream_t s) {
@@ -155,6 +160,7 @@ hipError_t run_grouped(int mode, int cfg, const void* table, int count, int tota
     case kF32: return run_grouped_f32(cfg, table, count, total_tiles, blocks, s);
     case kF64: return run_grouped_f64(cfg, table, count, total_tiles, blocks, s);
     case kBF16x3: return run_grouped_bf16x3(cfg, table, count, total_tiles, blocks, s);
+    case kBF16x4: return run_grouped_bf16x4(cfg, table, count, total_tiles, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -164,6 +170,7 @@ int grouped_occupancy(int mode, int cfg) {
     case kF32: return grouped_occupancy_f32(cfg);
     case kF64: return grouped_occupancy_f64(cfg);
     case kBF16x3: return grouped_occupancy_bf16x3(cfg);
+    case kBF16x4: return grouped_occupancy_bf16x4(cfg);
     default: return 1;
   }
 }
@@ -173,6 +180,7 @@ int num_tile_cfgs(int mode) {
     case kF32: return kF32NumTiles;
     case kF64: return kF64NumTiles;
     case kBF16x3: return kBF16x3NumTiles;
+    case kBF16x4: return kBF16x4NumTiles;
     default: return 0;
   }
 }
@@ -182,7 +190,8 @@ const TileInfo* tile_info(int mode, int cfg) {
   switch (mode) {
     case kF32: return &kF32Tiles[cfg];
     case kF64: return &kF64Tiles[cfg];
-    default: return &kBF16x3Tiles[cfg];
+    case kBF16x3: return &kBF16x3Tiles[cfg];
+    default: return &kBF16x4Tiles[cfg];
   }
 }
 

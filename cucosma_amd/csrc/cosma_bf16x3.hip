@@ -10,15 +10,21 @@ namespace cosma {
 using B3C0 = Tile<kBF16x3, 128, 128, 32, 64, 64>;
 using B3C1 = Tile<kBF16x3, 64, 64, 32, 32, 32>;
 using B3C2 = Tile<kBF16x3, 256, 128, 32, 64, 64>;
+using B3C3 = Tile<kBF16x3, 256, 256, 32, 128, 64>;
+using B3C4 = Tile<kBF16x3, 256, 128, 32, 128, 64>;
+using B3C5 = Tile<kBF16x3, 128, 128, 64, 64, 64>;
 
-extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info()};
-extern const int kBF16x3NumTiles = 3;
+extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info()};
+extern const int kBF16x3NumTiles = 6;
 
 hipError_t run_gemm_bf16x3(const Problem& p) {
   switch (p.cfg) {
     case 0: return B3C0::run(p);
     case 1: return B3C1::run(p);
     case 2: return B3C2::run(p);
+    case 3: return B3C3::run(p);
+    case 4: return B3C4::run(p);
+    case 5: return B3C5::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -29,6 +35,9 @@ hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, 
     case 0: return B3C0::grouped(table, count, total, blocks, s);
     case 1: return B3C1::grouped(table, count, total, blocks, s);
     case 2: return B3C2::grouped(table, count, total, blocks, s);
+    case 3: return B3C3::grouped(table, count, total, blocks, s);
+    case 4: return B3C4::grouped(table, count, total, blocks, s);
+    case 5: return B3C5::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -38,6 +47,9 @@ int grouped_occupancy_bf16x3(int cfg) {
     case 0: return B3C0::grouped_occupancy();
     case 1: return B3C1::grouped_occupancy();
     case 2: return B3C2::grouped_occupancy();
+    case 3: return B3C3::grouped_occupancy();
+    case 4: return B3C4::grouped_occupancy();
+    case 5: return B3C5::grouped_occupancy();
     default: return 1;
   }
 }

@@ -1,0 +1,58 @@
+// cosma_bf16x4.hip -- split-precision SGEMM with four products (bf16 hi/lo,
+// hi*hi + hi*lo + lo*hi + lo*lo on v_mfma_f32_16x16x32_bf16, FP32
+// accumulation): tile configurations, the same as cosma_bf16x3.hip.
+#include "cosma_launch.hpp"
+
+namespace cosma {
+
+// cfg 0: 128x128 workgroup tile, 4 waves of 64x64
+// cfg 1: 64x64, 4 waves of 32x32
+// cfg 2: 256x128, 8 waves of 64x64
+using B4C0 = Tile<kBF16x4, 128, 128, 32, 64, 64>;
+using B4C1 = Tile<kBF16x4, 64, 64, 32, 32, 32>;
+using B4C2 = Tile<kBF16x4, 256, 128, 32, 64, 64>;
+using B4C3 = Tile<kBF16x4, 256, 256, 32, 128, 64>;
+using B4C4 = Tile<kBF16x4, 256, 128, 32, 128, 64>;
+using B4C5 = Tile<kBF16x4, 128, 128, 64, 64, 64>;
+
+extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info()};
+extern const int kBF16x4NumTiles = 6;
+
+hipError_t run_gemm_bf16x4(const Problem& p) {
+  switch (p.cfg) {
+    case 0: return B4C0::run(p);
+    case 1: return B4C1::run(p);
+    case 2: return B4C2::run(p);
+    case 3: return B4C3::run(p);
+    case 4: return B4C4::run(p);
+    case 5: return B4C5::run(p);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, int total, int blocks,
+                         hipStream_t s) {
+  switch (cfg) {
+    case 0: return B4C0::grouped(table, count, total, blocks, s);
+    case 1: return B4C1::grouped(table, count, total, blocks, s);
+    case 2: return B4C2::grouped(table, count, total, blocks, s);
+    case 3: return B4C3::grouped(table, count, total, blocks, s);
+    case 4: return B4C4::grouped(table, count, total, blocks, s);
+    case 5: return B4C5::grouped(table, count, total, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int grouped_occupancy_bf16x4(int cfg) {
+  switch (cfg) {
+    case 0: return B4C0::grouped_occupancy();
+    case 1: return B4C1::grouped_occupancy();
+    case 2: return B4C2::grouped_occupancy();
+    case 3: return B4C3::grouped_occupancy();
+    case 4: return B4C4::grouped_occupancy();
+    case 5: return B4C5::grouped_occupancy();
+    default: return 1;
+  }
+}
+
+}  // namespace cosma

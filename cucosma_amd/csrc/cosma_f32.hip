@@ -11,9 +11,10 @@ using F32C0 = Tile<kF32, 128, 128, 32, 64, 64>;
 using F32C1 = Tile<kF32, 64, 64, 32, 32, 32>;
 using F32C2 = Tile<kF32, 128, 64, 32, 64, 32>;
 using F32C3 = Tile<kF32, 256, 128, 32, 64, 64>;
+using F32C4 = Tile<kF32, 128, 128, 64, 64, 64>;
 
-extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info()};
-extern const int kF32NumTiles = 4;
+extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info(), F32C4::info()};
+extern const int kF32NumTiles = 5;
 
 hipError_t run_gemm_f32(const Problem& p) {
   switch (p.cfg) {
@@ -21,6 +22,7 @@ hipError_t run_gemm_f32(const Problem& p) {
     case 1: return F32C1::run(p);
     case 2: return F32C2::run(p);
     case 3: return F32C3::run(p);
+    case 4: return F32C4::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -32,6 +34,7 @@ hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int
     case 1: return F32C1::grouped(table, count, total, blocks, s);
     case 2: return F32C2::grouped(table, count, total, blocks, s);
     case 3: return F32C3::grouped(table, count, total, blocks, s);
+    case 4: return F32C4::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -42,6 +45,7 @@ int grouped_occupancy_f32(int cfg) {
     case 1: return F32C1::grouped_occupancy();
     case 2: return F32C2::grouped_occupancy();
     case 3: return F32C3::grouped_occupancy();
+    case 4: return F32C4::grouped_occupancy();
     default: return 1;
   }
 }

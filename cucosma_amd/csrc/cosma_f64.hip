@@ -9,15 +9,19 @@ namespace cosma {
 using F64C0 = Tile<kF64, 128, 128, 16, 64, 64>;
 using F64C1 = Tile<kF64, 64, 64, 16, 32, 32>;
 using F64C2 = Tile<kF64, 128, 64, 16, 64, 32>;
+using F64C3 = Tile<kF64, 128, 128, 16, 64, 32>;
+using F64C4 = Tile<kF64, 64, 64, 32, 32, 32>;
 
-extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info()};
-extern const int kF64NumTiles = 3;
+extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info(), F64C3::info(), F64C4::info()};
+extern const int kF64NumTiles = 5;
 
 hipError_t run_gemm_f64(const Problem& p) {
   switch (p.cfg) {
     case 0: return F64C0::run(p);
     case 1: return F64C1::run(p);
     case 2: return F64C2::run(p);
+    case 3: return F64C3::run(p);
+    case 4: return F64C4::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -28,6 +32,8 @@ hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int
     case 0: return F64C0::grouped(table, count, total, blocks, s);
     case 1: return F64C1::grouped(table, count, total, blocks, s);
     case 2: return F64C2::grouped(table, count, total, blocks, s);
+    case 3: return F64C3::grouped(table, count, total, blocks, s);
+    case 4: return F64C4::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -37,6 +43,8 @@ int grouped_occupancy_f64(int cfg) {
     case 0: return F64C0::grouped_occupancy();
     case 1: return F64C1::grouped_occupancy();
     case 2: return F64C2::grouped_occupancy();
+    case 3: return F64C3::grouped_occupancy();
+    case 4: return F64C4::grouped_occupancy();
     default: return 1;
   }
 }

@@ -13,7 +13,7 @@ namespace cosma {
 // checked in cosma_api.cpp, split >= 1, and ws holds split*M*N accumulator
 // elements whenever split > 1.
 struct Problem {
-  int mode;  // 0 fp32, 1 fp64, 2 bf16x3
+  int mode;  // 0 fp32, 1 fp64, 2 bf16x3, 3 bf16x4
   int M, N, K;
   int ta, tb;
   const void* A;
@@ -60,6 +60,7 @@ const TileInfo* tile_info(int mode, int cfg);  // nullptr when out of range
 hipError_t run_gemm_f32(const Problem& p);
 hipError_t run_gemm_f64(const Problem& p);
 hipError_t run_gemm_bf16x3(const Problem& p);
+hipError_t run_gemm_bf16x4(const Problem& p);
 
 // Grouped persistent launch of `blocks` workgroups over `count` table
 // entries (device memory) holding `total_tiles` tiles of tile config cfg.

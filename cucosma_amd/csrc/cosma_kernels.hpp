@@ -6,6 +6,7 @@
 //   kBF16x3  FP32 operands split on the fly into bf16 hi + lo parts while the
 //            tile is written to LDS; hi*hi + hi*lo + lo*hi on
 //            v_mfma_f32_16x16x32_bf16 with FP32 accumulation
+//   kBF16x4  the same split with the fourth product lo*lo added
 //
 // Reference behaviour this replaces (Kernel/cuda/src/cuCOSMAV100.cuh):
 //   threadblock / warp / thread tiles     :2505-2565   -> workgroup / wave / MFMA sub-tiles
@@ -49,7 +50,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-enum Mode : int { kF32 = 0, kF64 = 1, kBF16x3 = 2 };
+enum Mode : int { kF32 = 0, kF64 = 1, kBF16x3 = 2, kBF16x4 = 3 };
+
+// Modes that split FP32 operands into bf16 hi/lo planes in LDS.
+constexpr bool is_split(int mode) { return mode == kBF16x3 || mode == kBF16x4; }
 
 template <int MODE>
 struct Traits;
@@ -96,6 +100,9 @@ struct Traits<kBF16x3> {
   static constexpr int PAD = 8;
   static constexpr int PLANES = 2;  // hi and lo
 };
+// Four products: the three of kBF16x3 plus lo*lo.
+template <>
+struct Traits<kBF16x4> : Traits<kBF16x3> {};
 
 template <int MODE>
 struct GemmParams {
@@ -166,7 +173,10 @@ struct TileLoader {
   static constexpr int GVEC = T::GVEC;
   static constexpr int LDK = BK + T::PAD;
   // KMAJ: units are 16-byte vectors along k.  Otherwise units are GVEC x GVEC
-  // blocks (GVEC k rows of one GVEC-wide vector along the row index).
+  // blocks (GVEC k rows of one GVEC-wide vector along the row index); the k
+  // block varies fastest across lanes, so the transposed LDS stores of
+  // neighbouring lanes fill one LDS row contiguously (no bank conflicts) while
+  // each global k row is still read in whole 128-byte lines.
   static constexpr int UNITS = KMAJ ? (R * BK / GVEC) : ((R / GVEC) * (BK / GVEC));
   static constexpr int PER = (UNITS + NT - 1) / NT;
   static constexpr int NREG = KMAJ ? PER : PER * GVEC;
@@ -206,13 +216,13 @@ struct TileLoader {
         v[i] = load_vec(p, gr < rmax, kend - gk, vec_ok);
       }
     } else {
-      constexpr int RB = R / GVEC;
+      constexpr int KB = BK / GVEC;
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int idx = tid + i * NT;
         if (!active(idx)) continue;
-        const int kb = idx / RB;
-        const int rb = idx - kb * RB;
+        const int rb = idx / KB;
+        const int kb = idx - rb * KB;
         const int gr = r0 + rb * GVEC;
 #pragma unroll
         for (int u = 0; u < GVEC; ++u) {
@@ -237,13 +247,13 @@ struct TileLoader {
         put(sh, sl, r * LDK + kv * GVEC, v[i]);
       }
     } else {
-      constexpr int RB = R / GVEC;
+      constexpr int KB = BK / GVEC;
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int idx = tid + i * NT;
         if (!active(idx)) continue;
-        const int kb = idx / RB;
-        const int rb = idx - kb * RB;
+        const int rb = idx / KB;
+        const int kb = idx - rb * KB;
 #pragma unroll
         for (int t = 0; t < GVEC; ++t) {
           GV w;
@@ -256,7 +266,7 @@ struct TileLoader {
   }
 
   __device__ __forceinline__ static void put(S* sh, S* sl, int off, GV w) {
-    if constexpr (MODE == kBF16x3) {
+    if constexpr (is_split(MODE)) {
       bf16x4 h, l;
       split_bf16x4(w, h, l);
       *reinterpret_cast<bf16x4*>(sh + off) = h;
@@ -341,7 +351,7 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
 #pragma unroll
   for (int q = 0; q < BK / Sh::KR; ++q) {
     const int kof = q * Sh::KR + g * T::KPL;
-    if constexpr (MODE == kBF16x3) {
+    if constexpr (is_split(MODE)) {
       const typename T::S* sal = st + Sh::PLANE;
       const typename T::S* sbl = sal + BM * LDK;
       Frag ah[TM], al[TM], bh[TN], bl[TN];
@@ -357,15 +367,47 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
         bh[j] = *reinterpret_cast<const Frag*>(sb + off);
         bl[j] = *reinterpret_cast<const Frag*>(sbl + off);
       }
-      // Small cross terms first, then the leading hi*hi term.
+      // Smallest terms first, then the leading hi*hi term.  kBF16x3 chains
+      // the three products into the running accumulator.  kBF16x4 sums its
+      // four products of this k-step in a zeroed temporary and adds that to
+      // the accumulator with one round-to-nearest VALU add, so the large
+      // running sum is rounded once per k-step instead of four times.
+      // Within one row of sub-tiles the product index is the outer loop, so
+      // TN independent MFMAs separate two that update the same accumulator.
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+      for (int i = 0; i < TM; ++i) {
+        if constexpr (MODE == kBF16x4) {
+          typename T::Acc t[TN];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < TN; ++j) t[j] = zero_acc<MODE>();
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl[j], t[j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], t[j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], t[j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], t[j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] += t[j][r];
+        } else {
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
+      }
     } else {
       Frag a[TM], b[TN];
 #pragma unroll

@@ -13,7 +13,8 @@ import torch
 
 from ._lib import MODES, GemmDesc, check, lib
 
-_DTYPE = {"fp32": torch.float32, "bf16x3": torch.float32, "fp64": torch.float64}
+_DTYPE = {"fp32": torch.float32, "bf16x3": torch.float32, "bf16x4": torch.float32,
+          "fp64": torch.float64}
 
 
 def _stream_ptr(stream) -> int:
@@ -50,8 +51,9 @@ def gemm(
 
     op(A) is A (shape M x K) or, with trans_a, A.T of a stored K x M tensor;
     op(B) is B (K x N) or, with trans_b, B.T of a stored N x K tensor.
-    mode: "fp32" (FP32 MFMA), "fp64" (FP64 MFMA) or "bf16x3" (FP32 operands,
-    bf16 hi/lo split, three bf16 MFMA products, FP32 accumulation).
+    mode: "fp32" (FP32 MFMA), "fp64" (FP64 MFMA), "bf16x3" (FP32 operands,
+    bf16 hi/lo split, three bf16 MFMA products, FP32 accumulation) or
+    "bf16x4" (the same split with the lo*lo product added).
     split_k <= 0 and tile_cfg < 0 select the library heuristic.
     """
     if mode not in MODES:
@@ -109,6 +111,10 @@ def workspace_bytes(m: int, n: int, k: int, *, mode: str = "fp32", split_k: int 
 
 def sgemm(A, B, C_out=None, alpha=1.0, beta=0.0, **kw):
     return gemm(A, B, C_out, alpha=alpha, beta=beta, mode="fp32", **kw)
+
+
+def sgemm_bf16x4(A, B, C_out=None, alpha=1.0, beta=0.0, **kw):
+    return gemm(A, B, C_out, alpha=alpha, beta=beta, mode="bf16x4", **kw)
 
 
 def sgemm_bf16x3(A, B, C_out=None, alpha=1.0, beta=0.0, **kw):

@@ -54,12 +54,13 @@ extern "C" {
 #define COSMA_MODE_FP32 0   /* FP32 in, FP32 MFMA, FP32 out */
 #define COSMA_MODE_FP64 1   /* FP64 in, FP64 MFMA, FP64 out */
 #define COSMA_MODE_BF16X3 2 /* FP32 in, bf16 hi/lo split, 3 bf16 MFMA products, FP32 out */
+#define COSMA_MODE_BF16X4 3 /* FP32 in, bf16 hi/lo split, 4 products (adds lo*lo), FP32 out */
 
 /* Full description of one GEMM: C = alpha * op(A) * op(B) + beta * C with
  * op(A) M x K and op(B) K x N.  trans_a == 0: A(i,k) = a[i*lda + k];
  * trans_a == 1: A(i,k) = a[k*lda + i].  trans_b == 0: B(k,j) = b[k*ldb + j];
  * trans_b == 1: B(k,j) = b[j*ldb + k].  trans_a == trans_b == 1 is not built.
- * Element type: float for FP32 and BF16X3, double for FP64. */
+ * Element type: float for FP32, BF16X3 and BF16X4, double for FP64. */
 typedef struct cosma_gemm_desc {
     int mode;
     int m, n, k;

@@ -49,6 +49,7 @@ def _load():
         "orc_gemm_f64_d": (None, [i, i, i, d, vp, i, i, vp, i, i, d, vp, i, vp, i, i, i]),
         "orc_gemm_ld": (None, [i, i, i, d, vp, i, i, vp, i, i, d, vp, i, vp, i, i, i]),
         "orc_gemm_bf16x3_f64": (None, [i, i, i, d, vp, i, i, vp, i, i, d, vp, i, vp, i, i, i]),
+        "orc_gemm_bf16x4_f64": (None, [i, i, i, d, vp, i, i, vp, i, i, d, vp, i, vp, i, i, i]),
         "orc_refsim_sgemm": (None, [i, i, i, f, f, vp, i, vp, i, vp, i, i, i, i, i, i]),
         "orc_cpu_naive_sgemm": (None, [i, i, i, vp, vp, vp, i, i, i]),
         "orc_omp_max_threads": (i, []),
@@ -164,6 +165,11 @@ def gemm_ld(A, B, C0=None, alpha=1.0, beta=0.0, ta=False, tb=False, rows=None, r
 def gemm_bf16x3(A, B, C0=None, alpha=1.0, beta=0.0, ta=False, tb=False, rows=None, row0=0):
     """What the three-product bf16 split computes, with exact (double) accumulation."""
     return _gemm(lib.orc_gemm_bf16x3_f64, np.float32, A, B, C0, alpha, beta, ta, tb, rows, row0)
+
+
+def gemm_bf16x4(A, B, C0=None, alpha=1.0, beta=0.0, ta=False, tb=False, rows=None, row0=0):
+    """The four-product bf16 split (adds lo*lo), with exact (double) accumulation."""
+    return _gemm(lib.orc_gemm_bf16x4_f64, np.float32, A, B, C0, alpha, beta, ta, tb, rows, row0)
 
 
 def refsim_sgemm(A, B, C0, alpha, beta, split_k=1, load_k=8, tbk=None, fused_beta=True,

@@ -41,6 +41,8 @@
  *   orc_cpu_naive_sgemm         bit-exact vs a numpy float32 loop without fma;
  *                               threaded == serial
  *   orc_bf16_*                  RNE vs torch.bfloat16 conversion; closed forms
+ *   orc_gemm_bf16x{3,4}_f64     exact on 8-bit integers; bounds vs orc_gemm_f64;
+ *                               dropped-term planted checks
  *   orc_err_*                   vs numpy.linalg.norm
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off -shared).
@@ -401,13 +403,13 @@ void orc_split_bf16(int64_t n, const float *x, uint16_t *hi, uint16_t *lo) {
     }
 }
 
-/* What the three-product split-precision GEMM computes, accumulated exactly
- * enough (double) that only the split itself is modelled:
- * out = alpha * sum_k (ah*bh + ah*bl + al*bh) + beta*C0.  Each bf16 product is
- * exact in float and in double.  Rows [row0, row0+rows). */
-void orc_gemm_bf16x3_f64(int M, int N, int K, double alpha, const float *A, int lda, int ta,
+/* What the split-precision GEMM computes, accumulated exactly enough
+ * (double) that only the split itself is modelled:
+ * out = alpha * sum_k (ah*bh + ah*bl + al*bh [+ al*bl when nprod == 4]) + beta*C0.
+ * Each bf16 product is exact in float and in double.  Rows [row0, row0+rows). */
+void orc_gemm_bf16xn_f64(int M, int N, int K, double alpha, const float *A, int lda, int ta,
                          const float *B, int ldb, int tb, double beta, const float *C0,
-                         int ldc, double *out, int ldo, int row0, int rows) {
+                         int ldc, double *out, int ldo, int row0, int rows, int nprod) {
     (void)M;
     float *bh = (float *)malloc(sizeof(float) * (size_t)K * N);
     float *bl = (float *)malloc(sizeof(float) * (size_t)K * N);
@@ -418,6 +420,7 @@ void orc_gemm_bf16x3_f64(int M, int N, int K, double alpha, const float *A, int 
             bh[(int64_t)k * N + j] = h;
             bl[(int64_t)k * N + j] = orc_bf16_to_f32(orc_bf16_rne(x - h));
         }
+    const double w4 = nprod == 4 ? 1.0 : 0.0;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int ii = 0; ii < rows; ++ii) {
         int i = row0 + ii;
@@ -429,7 +432,8 @@ void orc_gemm_bf16x3_f64(int M, int N, int K, double alpha, const float *A, int 
             float l = orc_bf16_to_f32(orc_bf16_rne(x - h));
             const float *bhr = bh + (int64_t)k * N, *blr = bl + (int64_t)k * N;
             for (int j = 0; j < N; ++j)
-                o[j] += (double)h * bhr[j] + (double)h * blr[j] + (double)l * bhr[j];
+                o[j] += (double)h * bhr[j] + (double)h * blr[j] + (double)l * bhr[j] +
+                        w4 * ((double)l * blr[j]);
         }
         for (int j = 0; j < N; ++j) {
             double v = alpha * o[j];
@@ -439,6 +443,19 @@ void orc_gemm_bf16x3_f64(int M, int N, int K, double alpha, const float *A, int 
     }
     free(bh);
     free(bl);
+}
+
+void orc_gemm_bf16x3_f64(int M, int N, int K, double alpha, const float *A, int lda, int ta,
+                         const float *B, int ldb, int tb, double beta, const float *C0,
+                         int ldc, double *out, int ldo, int row0, int rows) {
+    orc_gemm_bf16xn_f64(M, N, K, alpha, A, lda, ta, B, ldb, tb, beta, C0, ldc, out, ldo, row0,
+                        rows, 3);
+}
+void orc_gemm_bf16x4_f64(int M, int N, int K, double alpha, const float *A, int lda, int ta,
+                         const float *B, int ldb, int tb, double beta, const float *C0,
+                         int ldc, double *out, int ldo, int row0, int rows) {
+    orc_gemm_bf16xn_f64(M, N, K, alpha, A, lda, ta, B, ldb, tb, beta, C0, ldc, out, ldo, row0,
+                        rows, 4);
 }
 
 /* ------------------------------------------------------------------------ */

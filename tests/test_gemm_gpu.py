@@ -42,7 +42,7 @@ def _int_operands(rng, M, N, K, dt, lo=-8, hi=9):
     return A, B, C0
 
 
-MODES = [("fp32", np.float32), ("fp64", np.float64), ("bf16x3", np.float32)]
+MODES = [("fp32", np.float32), ("fp64", np.float64), ("bf16x3", np.float32), ("bf16x4", np.float32)]
 LAYOUTS = [(False, False), (False, True), (True, False)]
 
 
@@ -106,12 +106,13 @@ def test_fp32_random_vs_f64(cc, dev, ta, tb, shape):
     _bound_check(Cd.cpu().numpy(), ref, A, B, 2.0, 2.0**-24)
 
 
-def test_bf16x3_vs_split_oracle_and_f64(cc, dev):
+@pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
+def test_split_modes_vs_split_oracle_and_f64(cc, dev, mode):
     M, N, K = 300, 200, 1000
     A, B = oracle.workload_square_flat(M, N, K, seed=1)
-    got = cc.gemm(_t(A, dev), _t(B, dev), mode="bf16x3").cpu().numpy()
-    exact3 = oracle.gemm_bf16x3(A, B)  # hi*hi + hi*lo + lo*hi accumulated in double
-    _bound_check(got, exact3, A, B, 2.0, 2.0**-24)
+    got = cc.gemm(_t(A, dev), _t(B, dev), mode=mode).cpu().numpy()
+    exact = oracle.gemm_bf16x3(A, B) if mode == "bf16x3" else oracle.gemm_bf16x4(A, B)
+    _bound_check(got, exact, A, B, 2.0, 2.0**-24)  # only FP32 accumulation error remains
     ref = oracle.gemm_f64(A, B)
     _bound_check(got, ref, A, B, 2.0, 2.0**-24, extra=2.0**-15)
 
@@ -145,13 +146,13 @@ def test_splitk_deterministic_and_consistent(cc, dev):
     A, B = oracle.workload_square_flat(M, N, K, seed=1)
     Ad, Bd = _t(A, dev), _t(B, dev)
     ref = oracle.gemm_f64(A, B)
-    for mode in ("fp32", "bf16x3"):
+    for mode in ("fp32", "bf16x3", "bf16x4"):
         for split in (2, 7, 16):
             first = cc.gemm(Ad, Bd, mode=mode, split_k=split, tile_cfg=1).cpu().numpy()
             for _ in range(3):
                 again = cc.gemm(Ad, Bd, mode=mode, split_k=split, tile_cfg=1).cpu().numpy()
                 assert np.array_equal(first.view(np.uint32), again.view(np.uint32))
-            extra = 2.0**-15 if mode == "bf16x3" else 0.0
+            extra = 2.0**-15 if mode.startswith("bf16") else 0.0
             _bound_check(first, ref, A, B, 2.0, 2.0**-24, extra)
 
 

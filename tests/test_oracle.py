@@ -314,6 +314,16 @@ def test_gemm_bf16x3_oracle():
     hiB = oracle.bf16_bits_to_f32(oracle.split_bf16(B)[0])
     hh = hiA.astype(np.float64) @ hiB.astype(np.float64)
     assert np.abs(hh - ref).max() > 50 * np.abs(got - ref).max()
+    # four products: exactly (ah+al)(bh+bl); strictly closer to FP64 in norm,
+    # and differs from the three-product sum by exactly al*bl
+    got4 = oracle.gemm_bf16x4(A, B)
+    loA = oracle.bf16_bits_to_f32(oracle.split_bf16(A)[1]).astype(np.float64)
+    loB = oracle.bf16_bits_to_f32(oracle.split_bf16(B)[1]).astype(np.float64)
+    # (differences of two double sums of 256 terms: absolute tolerance)
+    assert np.allclose(got4 - got, loA @ loB, rtol=0, atol=1e-10)
+    assert np.linalg.norm(got4 - ref) < np.linalg.norm(got - ref)
+    full = (hiA.astype(np.float64) + loA) @ (hiB.astype(np.float64) + loB)
+    assert np.allclose(got4, full, rtol=1e-12, atol=1e-15)
 
 
 # ---------------------------------------------------------------- metrics
