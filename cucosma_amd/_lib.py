@@ -1,6 +1,6 @@
 """ctypes binding of libcosma_amd.so (the C ABI in include/cosma/cosma.h).
 
-The library is built in-tree by ``python -m cucosma_amd.build``.  Importing
+The library is built in-tree by ``python cucosma_amd/build.py``.  Importing
 this module fails loudly when it is missing: there is no fallback path.
 """
 from __future__ import annotations
@@ -81,7 +81,7 @@ class GroupProblem(C.Structure):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing: build the HIP library first (python -m cucosma_amd.build)"
+            f"{LIB_PATH} is missing: build the HIP library first (python cucosma_amd/build.py)"
         )
     lib = C.CDLL(LIB_PATH)
     f32p, f64p, vp = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_void_p
@@ -120,6 +120,10 @@ def _load():
         "cosma_grouped_plan_bucket": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i),
                                           C.POINTER(i)]),
         "cosma_grouped_plan_destroy": (i, [vp]),
+        "cosma_benchmark": (i, [C.POINTER(GemmDesc), i, i, C.c_size_t, C.POINTER(f)]),
+        "cosma_tune_set": (i, [i] * 9),
+        "cosma_tune_lookup": (i, [i] * 6 + [C.POINTER(i)] * 3),
+        "cosma_tune_clear": (i, []),
         "cosma_status_string": (C.c_char_p, [i]),
         "cosma_version": (C.c_char_p, []),
     }

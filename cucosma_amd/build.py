@@ -1,7 +1,8 @@
 """Build libcosma_amd.so (the HIP kernels + C ABI) in-tree for gfx950.
 
 Runs hipcc directly, one process per translation unit, then links the shared
-library next to this file.  Usable as ``python -m cucosma_amd.build``.
+library next to this file.  Run it by path, ``python cucosma_amd/build.py``,
+so the package (which loads the library on import) is not imported first.
 """
 from __future__ import annotations
 

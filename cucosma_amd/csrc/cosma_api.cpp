@@ -8,7 +8,9 @@
 // returns a negative status without touching the GPU.
 #include <hip/hip_runtime_api.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "../../include/cosma/cosma.h"
@@ -159,6 +161,22 @@ int resolve_workspace(const cosma_gemm_desc* d, size_t need, hipStream_t s, void
   return 0;
 }
 
+// Tuned choices installed by cosma_tune_set (or the Python autotuner's DB).
+using TuneKey = std::tuple<int, int, int, int, int, int>;
+struct TuneVal {
+  int cfg, split, group_m;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+bool tuned(int mode, int m, int n, int k, int ta, int tb, TuneVal* out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(TuneKey{mode, m, n, k, ta, tb});
+  if (it == g_tune.end()) return false;
+  *out = it->second;
+  return true;
+}
+
 // Host-side contract check; returns 0, or a negative status.
 int validate(const cosma_gemm_desc* d) {
   if (!d) return COSMA_STATUS_INVALID_VALUE;
@@ -199,6 +217,14 @@ int run(const cosma_gemm_desc* d) {
     return (int)e;
   }
   Choice ch = heuristic(d->mode, d->m, d->n, d->k);
+  int group_m = d->group_m;
+  TuneVal tv;
+  if (d->tile_cfg < 0 && d->split_k <= 0 &&
+      tuned(d->mode, d->m, d->n, d->k, d->trans_a, d->trans_b, &tv)) {
+    ch.cfg = tv.cfg;
+    ch.split = tv.split;
+    if (group_m <= 0) group_m = tv.group_m;
+  }
   Problem p;
   p.mode = d->mode;
   p.M = d->m;
@@ -217,7 +243,7 @@ int run(const cosma_gemm_desc* d) {
   p.cfg = d->tile_cfg >= 0 ? d->tile_cfg : ch.cfg;
   p.split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
   if (p.split > d->k) p.split = d->k > 0 ? d->k : 1;
-  p.group_m = d->group_m;
+  p.group_m = group_m;
   p.ws = nullptr;
   p.stream = s;
   if (p.split > 1) {
@@ -515,6 +541,10 @@ int cosma_workspace_query(const cosma_gemm_desc* d, size_t* bytes) {
   *bytes = 0;
   if (d->m == 0 || d->n == 0 || d->k == 0 || d->alpha == 0.0) return 0;
   Choice ch = heuristic(d->mode, d->m, d->n, d->k);
+  TuneVal tv;
+  if (d->tile_cfg < 0 && d->split_k <= 0 &&
+      tuned(d->mode, d->m, d->n, d->k, d->trans_a, d->trans_b, &tv))
+    ch.split = tv.split;
   int split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
   if (split > d->k) split = d->k;
   if (split > 1) *bytes = (size_t)split * (size_t)d->m * (size_t)d->n * elem_size(d->mode);
@@ -664,6 +694,174 @@ int cosma_convert_f32_f64(int rows, int cols, const float* X, int64_t ldx, doubl
   if (rows == 0 || cols == 0) return 0;
   return (int)cosma::run_convert_f32_f64(rows, cols, X, ldx, Y, ldy,
                                          static_cast<hipStream_t>(stream));
+}
+
+int cosma_tune_set(int mode, int m, int n, int k, int trans_a, int trans_b, int tile_cfg,
+                   int split_k, int group_m) {
+  if (mode < 0 || mode >= kNumModes || m < 0 || n < 0 || k < 0 || (trans_a != 0 && trans_a != 1) ||
+      (trans_b != 0 && trans_b != 1) || tile_cfg < 0 || tile_cfg >= kNumCfg[mode] || split_k < 1 ||
+      split_k > kMaxSplit || group_m < 0)
+    return COSMA_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[TuneKey{mode, m, n, k, trans_a, trans_b}] = TuneVal{tile_cfg, split_k, group_m};
+  return 0;
+}
+
+int cosma_tune_lookup(int mode, int m, int n, int k, int trans_a, int trans_b, int* tile_cfg,
+                      int* split_k, int* group_m) {
+  if (!tile_cfg || !split_k || !group_m) return COSMA_STATUS_INVALID_VALUE;
+  TuneVal tv;
+  if (!tuned(mode, m, n, k, trans_a, trans_b, &tv)) return COSMA_STATUS_NOT_SUPPORTED;
+  *tile_cfg = tv.cfg;
+  *split_k = tv.split;
+  *group_m = tv.group_m;
+  return 0;
+}
+
+int cosma_tune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+  return 0;
+}
+
+int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_result* results,
+                   int max_results, int* n_results, int* best) {
+  int st = validate(d);
+  if (st != 0) return st;
+  if (!n_results || !best || (max_results > 0 && !results) || warmup < 0 || reps < 1)
+    return COSMA_STATUS_INVALID_VALUE;
+  *n_results = 0;
+  *best = -1;
+  if (d->m == 0 || d->n == 0 || d->k == 0 || d->alpha == 0.0) return 0;
+  const double peak[kNumModes] = {157.3, 78.6, 2516.6 / 3.0, 2516.6 / 4.0};
+  hipStream_t s = static_cast<hipStream_t>(d->stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    if (e0) hipEventDestroy(e0);
+    return (int)he;
+  }
+  const double flops = 2.0 * d->m * (double)d->n * d->k;
+  std::vector<cosma_tune_result> all;
+  auto try_one = [&](int cfg, int split, int gm) {
+    cosma_gemm_desc c = *d;
+    c.tile_cfg = cfg;
+    c.split_k = split;
+    c.group_m = gm;
+    cosma_tune_result r{};
+    r.tile_cfg = cfg;
+    r.split_k = split;
+    r.group_m = gm;
+    size_t ws = 0;
+    cosma_workspace_query(&c, &ws);
+    r.workspace_bytes = ws;
+    const TileDims td = kDims[d->mode][cfg];
+    r.blocks = tiles_of(d->mode, cfg, d->m, d->n) * split;
+    const bool edge = (d->m % td.bm) || (d->n % td.bn) || (d->k % td.bk);
+    r.occupancy = cosma::gemm_occupancy(d->mode, cfg, d->trans_a, d->trans_b, edge);
+    const cosma::TileInfo* ti = cosma::tile_info(d->mode, cfg);
+    r.waves_per_simd = ti ? r.occupancy * (ti->threads / 64) / 4 : 0;
+    int status = 0;
+    for (int i = 0; i < warmup && status == 0; ++i) status = run(&c);
+    if (status == 0) status = (int)hipEventRecord(e0, s);
+    for (int i = 0; i < reps && status == 0; ++i) status = run(&c);
+    if (status == 0) status = (int)hipEventRecord(e1, s);
+    if (status == 0) status = (int)hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (status == 0) status = (int)hipEventElapsedTime(&ms, e0, e1);
+    r.status = status;
+    if (status == 0) {
+      r.ms = ms / reps;
+      r.tflops = r.ms > 0 ? flops / (r.ms * 1e-3) / 1e12 : 0.0;
+      r.mfma_util = r.tflops / peak[d->mode];
+    }
+    all.push_back(r);
+  };
+  // Pass 1: tile configuration x split-K at the default group size.
+  const int cfg_lo = d->tile_cfg >= 0 ? d->tile_cfg : 0;
+  const int cfg_hi = d->tile_cfg >= 0 ? d->tile_cfg + 1 : kNumCfg[d->mode];
+  const int gm0 = d->group_m > 0 ? d->group_m : 8;
+  for (int cfg = cfg_lo; cfg < cfg_hi; ++cfg) {
+    const TileDims td = kDims[d->mode][cfg];
+    const int tiles = tiles_of(d->mode, cfg, d->m, d->n);
+    for (int split = 1; split <= 64; split *= 2) {
+      if (d->split_k > 0 && split != d->split_k) continue;
+      if (split > 1 && (split * 2 * td.bk > d->k || (int64_t)tiles * split > 16 * kCUs)) continue;
+      try_one(cfg, split, gm0);
+    }
+    if (d->split_k > 64) try_one(cfg, d->split_k, gm0);
+  }
+  auto best_of = [&]() {
+    int b = -1;
+    for (int i = 0; i < (int)all.size(); ++i)
+      if (all[i].status == 0 && (b < 0 || all[i].ms < all[b].ms)) b = i;
+    return b;
+  };
+  // Pass 2: XCD group size for the best (cfg, split) when the grid has
+  // enough tile rows for grouping to matter.
+  int b = best_of();
+  if (b >= 0 && d->group_m <= 0) {
+    const int cfg = all[b].tile_cfg, split = all[b].split_k;
+    if (cdiv(d->m, kDims[d->mode][cfg].bm) > 8)
+      for (int gm : {2, 4, 16, 32}) try_one(cfg, split, gm);
+  }
+  he = hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *best = best_of();
+  const int n = (int)all.size() < max_results ? (int)all.size() : max_results;
+  for (int i = 0; i < n; ++i) results[i] = all[i];
+  *n_results = n;
+  if (*best >= n) *best = -1;
+  return 0;
+}
+
+int cosma_benchmark(const cosma_gemm_desc* d, int warmup, int runs, size_t flush_bytes,
+                    float* ms_out) {
+  int st = validate(d);
+  if (st != 0) return st;
+  if (warmup < 0 || runs < 0 || (runs > 0 && !ms_out)) return COSMA_STATUS_INVALID_VALUE;
+  hipStream_t s = static_cast<hipStream_t>(d->stream);
+  void* flush = nullptr;
+  if (flush_bytes > 0) {
+    static std::mutex fmu;
+    static void* fbuf = nullptr;
+    static size_t fbytes = 0;
+    std::lock_guard<std::mutex> lk(fmu);
+    if (fbytes < flush_bytes) {
+      if (capturing(s)) return COSMA_STATUS_WORKSPACE;
+      if (fbuf) hipFree(fbuf);
+      fbuf = nullptr;
+      fbytes = 0;
+      hipError_t e = hipMalloc(&fbuf, flush_bytes);
+      if (e != hipSuccess) return (int)e;
+      fbytes = flush_bytes;
+    }
+    flush = fbuf;
+  }
+  for (int i = 0; i < warmup; ++i) {
+    st = run(d);
+    if (st) return st;
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return (int)e;
+  hipEvent_t e0, e1;
+  if ((e = hipEventCreate(&e0)) != hipSuccess) return (int)e;
+  if ((e = hipEventCreate(&e1)) != hipSuccess) {
+    hipEventDestroy(e0);
+    return (int)e;
+  }
+  for (int i = 0; i < runs && st == 0; ++i) {
+    if (flush) st = (int)cosma::run_flush(flush, flush_bytes, s);
+    if (st == 0) st = (int)hipEventRecord(e0, s);
+    if (st == 0) st = run(d);
+    if (st == 0) st = (int)hipEventRecord(e1, s);
+    if (st == 0) st = (int)hipEventSynchronize(e1);
+    if (st == 0) st = (int)hipEventElapsedTime(&ms_out[i], e0, e1);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return st;
 }
 
 const char* cosma_status_string(int status) {

@@ -175,6 +175,21 @@ int grouped_occupancy(int mode, int cfg) {
   }
 }
 
+int gemm_occupancy_f32(int cfg, int ta, int tb, bool edge);
+int gemm_occupancy_f64(int cfg, int ta, int tb, bool edge);
+int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge);
+int gemm_occupancy_bf16x4(int cfg, int ta, int tb, bool edge);
+
+int gemm_occupancy(int mode, int cfg, int ta, int tb, bool edge) {
+  switch (mode) {
+    case kF32: return gemm_occupancy_f32(cfg, ta, tb, edge);
+    case kF64: return gemm_occupancy_f64(cfg, ta, tb, edge);
+    case kBF16x3: return gemm_occupancy_bf16x3(cfg, ta, tb, edge);
+    case kBF16x4: return gemm_occupancy_bf16x4(cfg, ta, tb, edge);
+    default: return 0;
+  }
+}
+
 int num_tile_cfgs(int mode) {
   switch (mode) {
     case kF32: return kF32NumTiles;
@@ -222,6 +237,20 @@ hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N
                           double alpha, double beta, hipStream_t s) {
   hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(grid_for((int64_t)M * N, 256, 8192)),
                      dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta);
+  return hipGetLastError();
+}
+
+// Cache flush: overwrite a buffer larger than L2 + Infinity Cache.
+__global__ void __launch_bounds__(256) flush_kernel(float4* __restrict__ buf, int64_t n, float v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    buf[i] = make_float4(v, v, v, v);
+}
+
+hipError_t run_flush(void* buf, size_t bytes, hipStream_t s) {
+  const int64_t n = (int64_t)(bytes / sizeof(float4));
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(flush_kernel, dim3(4096), dim3(256), 0, s, static_cast<float4*>(buf), n, 1.0f);
   return hipGetLastError();
 }
 

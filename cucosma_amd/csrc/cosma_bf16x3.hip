@@ -54,4 +54,16 @@ int grouped_occupancy_bf16x3(int cfg) {
   }
 }
 
+int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge) {
+  switch (cfg) {
+    case 0: return B3C0::occupancy(ta, tb, edge);
+    case 1: return B3C1::occupancy(ta, tb, edge);
+    case 2: return B3C2::occupancy(ta, tb, edge);
+    case 3: return B3C3::occupancy(ta, tb, edge);
+    case 4: return B3C4::occupancy(ta, tb, edge);
+    case 5: return B3C5::occupancy(ta, tb, edge);
+    default: return 0;
+  }
+}
+
 }  // namespace cosma

@@ -55,4 +55,16 @@ int grouped_occupancy_bf16x4(int cfg) {
   }
 }
 
+int gemm_occupancy_bf16x4(int cfg, int ta, int tb, bool edge) {
+  switch (cfg) {
+    case 0: return B4C0::occupancy(ta, tb, edge);
+    case 1: return B4C1::occupancy(ta, tb, edge);
+    case 2: return B4C2::occupancy(ta, tb, edge);
+    case 3: return B4C3::occupancy(ta, tb, edge);
+    case 4: return B4C4::occupancy(ta, tb, edge);
+    case 5: return B4C5::occupancy(ta, tb, edge);
+    default: return 0;
+  }
+}
+
 }  // namespace cosma

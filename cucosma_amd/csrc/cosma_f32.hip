@@ -50,4 +50,15 @@ int grouped_occupancy_f32(int cfg) {
   }
 }
 
+int gemm_occupancy_f32(int cfg, int ta, int tb, bool edge) {
+  switch (cfg) {
+    case 0: return F32C0::occupancy(ta, tb, edge);
+    case 1: return F32C1::occupancy(ta, tb, edge);
+    case 2: return F32C2::occupancy(ta, tb, edge);
+    case 3: return F32C3::occupancy(ta, tb, edge);
+    case 4: return F32C4::occupancy(ta, tb, edge);
+    default: return 0;
+  }
+}
+
 }  // namespace cosma

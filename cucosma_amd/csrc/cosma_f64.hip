@@ -49,4 +49,15 @@ int grouped_occupancy_f64(int cfg) {
   }
 }
 
+int gemm_occupancy_f64(int cfg, int ta, int tb, bool edge) {
+  switch (cfg) {
+    case 0: return F64C0::occupancy(ta, tb, edge);
+    case 1: return F64C1::occupancy(ta, tb, edge);
+    case 2: return F64C2::occupancy(ta, tb, edge);
+    case 3: return F64C3::occupancy(ta, tb, edge);
+    case 4: return F64C4::occupancy(ta, tb, edge);
+    default: return 0;
+  }
+}
+
 }  // namespace cosma

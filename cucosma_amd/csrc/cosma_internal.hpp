@@ -68,6 +68,9 @@ hipError_t run_grouped(int mode, int cfg, const void* table, int count, int tota
                        hipStream_t s);
 // Workgroups of the grouped kernel of (mode, cfg) that fit on one CU.
 int grouped_occupancy(int mode, int cfg);
+// Workgroups of one gemm_kernel instantiation that fit on one CU (0 if the
+// query fails or the layout is not built).
+int gemm_occupancy(int mode, int cfg, int ta, int tb, bool edge);
 
 // C = beta*C over the M x N window (alpha == 0 or K == 0).
 hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, hipStream_t s);
@@ -78,6 +81,9 @@ hipError_t run_reduce_f32(const float* ws, float* C, int64_t ldc, int M, int N, 
                           float beta, hipStream_t s);
 hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N, int S,
                           double alpha, double beta, hipStream_t s);
+
+// Overwrite `bytes` of buf (cache flush between timed runs).
+hipError_t run_flush(void* buf, size_t bytes, hipStream_t s);
 
 // FP64 statistics; part must hold kStatParts * 5 doubles, out 5 doubles (device).
 constexpr int kStatParts = 1024;

@@ -37,6 +37,23 @@ struct Tile {
     return n > 0 ? n : 1;
   }
 
+  // Workgroups of one gemm_kernel instantiation that fit on a CU.
+  template <bool TA, bool TB, bool EDGE>
+  static int occ() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(&gemm_kernel<MODE, BM, BN, BK, WM, WN, TA, TB, EDGE>),
+            Sh::NT, 0) != hipSuccess)
+      return 0;
+    return n;
+  }
+  static int occupancy(int ta, int tb, bool edge) {
+    if (ta == 0 && tb == 0) return edge ? occ<false, false, true>() : occ<false, false, false>();
+    if (ta == 0 && tb == 1) return edge ? occ<false, true, true>() : occ<false, true, false>();
+    if (ta == 1 && tb == 0) return edge ? occ<true, false, true>() : occ<true, false, false>();
+    return 0;
+  }
+
   static hipError_t run(const Problem& pr) {
     using G = typename Traits<MODE>::G;
     using E = typename Traits<MODE>::E;

@@ -159,6 +159,52 @@ int cosma_sumsq_f64(int rows, int cols, const double *X, int64_t ldx, double *ou
 int cosma_convert_f32_f64(int rows, int cols, const float *X, int64_t ldx, double *Y, int64_t ldy,
                           void *stream);
 
+/* ---- Autotuner: replaces ScheduleGenerator's communication-volume model
+ * (ScheduleGenerator/ScheduleGeneratorv4.h:13-391, main.cpp:22-342) with
+ * measurement.  Candidates are every built tile configuration x split-K
+ * factor x XCD group size that fits the shape; each is timed with HIP events
+ * on the descriptor's stream and reported with its occupancy and MFMA
+ * utilisation.  The fastest is the choice. ---- */
+typedef struct cosma_tune_result {
+    int tile_cfg, split_k, group_m;
+    int status;        /* 0, or the status of the failed launch */
+    double ms;         /* mean time per GEMM (HIP events over reps calls) */
+    double tflops;     /* 2*M*N*K / ms */
+    double mfma_util;  /* tflops / the mode's dense MFMA peak (FP32 157.3, FP64 78.6,
+                          BF16X3 2516.6/3, BF16X4 2516.6/4 TFLOP/s) */
+    int blocks;        /* workgroups of the main kernel */
+    int occupancy;     /* workgroups per CU the kernel can hold */
+    int waves_per_simd;
+    size_t workspace_bytes;
+} cosma_tune_result;
+
+/* Time the candidates for the GEMM *d (its operands are used and C is
+ * overwritten; d->tile_cfg / split_k / group_m >= 0 pin that parameter).
+ * Fills up to max_results entries, sets *n_results and *best (index of the
+ * fastest successful candidate, -1 if none).  Allocates split-K workspace as
+ * needed: do not call while capturing. */
+int cosma_autotune(const cosma_gemm_desc *d, int warmup, int reps, cosma_tune_result *results,
+                   int max_results, int *n_results, int *best);
+/* Tuned table consulted by every entry point whose descriptor leaves tile_cfg
+ * and split_k to the library: an exact (mode, m, n, k, trans_a, trans_b) match
+ * overrides the built-in heuristic. */
+int cosma_tune_set(int mode, int m, int n, int k, int trans_a, int trans_b, int tile_cfg,
+                   int split_k, int group_m);
+int cosma_tune_lookup(int mode, int m, int n, int k, int trans_a, int trans_b, int *tile_cfg,
+                      int *split_k, int *group_m);
+int cosma_tune_clear(void);
+
+/* Benchmark protocol of the reference harness (Kernel/hip/src/benchmark.h:39-91,
+ * Cacheflusher.h:18-49): `warmup` untimed calls, then `runs` calls each
+ * bracketed by a HIP event pair on the descriptor's stream, with the caches
+ * flushed between runs by overwriting flush_bytes of device memory (0: no
+ * flush; the library sizes 512 MiB so that both L2 and the 256 MB Infinity
+ * Cache are evicted, where the reference flushed an L2-sized buffer).
+ * ms_out[i] receives run i's event time.  Allocates the flush buffer on first
+ * use. */
+int cosma_benchmark(const cosma_gemm_desc *d, int warmup, int runs, size_t flush_bytes,
+                    float *ms_out);
+
 /* ---- Grouped GEMM: many independent row-major problems, one persistent
  * launch per tile-configuration bucket, replayable as one hipGraph.  No
  * reference counterpart: the reference builds one binary per shape
