@@ -83,6 +83,14 @@ def _load():
         raise ImportError(
             f"{LIB_PATH} is missing: build the HIP library first (python cucosma_amd/build.py)"
         )
+    # torch bundles its own libamdhip64 (file libamdhip64.so, soname
+    # libamdhip64.so.7).  Loading it first lets this library's
+    # libamdhip64.so.7 dependency resolve to that same runtime; loading
+    # /opt/rocm's first would put two HIP runtimes in one process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     f32p, f64p, vp = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_void_p
     i, i64, d, f = C.c_int, C.c_int64, C.c_double, C.c_float

@@ -60,37 +60,48 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
   return cdiv(m, kDims[mode][cfg].bm) * cdiv(n, kDims[mode][cfg].bn);
 }
 
-// Heuristic tile / split choice (replaced per shape by the autotuner's
-// measured choice when the caller passes tile_cfg / split_k).
-// Each mode has a large tile measured fastest at 8192^3 / 32768x1024x1024
-// (FP32 128x128 with 4 waves, FP64 128x128 with 8 waves of 64x32, BF16x3
-// 256x256 with 8 waves of 128x64, BF16x4 256x128 with 8 waves of 64x64, which
-// fits its extra temporaries without spilling); it is used while it still gives every CU
-// two workgroups.  Smaller outputs step down to 128x64 and then 64x64 tiles
-// and split K, keeping every K
-// slice at least 512 deep.  Split-K aims at four workgroups per CU (the
-// measured optimum on 256x256x1048576).
+// Heuristic tile / split choice, used when neither the caller nor the tuned
+// table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
+// MI355X:
+//  - large outputs (at least two workgroups per CU on the mode's large tile):
+//    FP32 128x128 (4 waves of 64x64), FP64 128x128 (8 waves of 64x32), BF16x3
+//    256x256 (8 waves of 128x64), BF16x4 256x128 (8 waves of 64x64; its
+//    per-k-step temporaries do not fit the 128x64 wave tile);
+//  - deep-K outputs (few tiles, K >= 16384): split K over a 128-row tile with
+//    a long K chunk -- 128x64x32 (FP32/FP64) or 128x128x64 (BF16) -- until
+//    every CU holds two (FP32/FP64) or one (BF16, 144 KiB of LDS) workgroups;
+//  - otherwise 128x128 (BF16) / 128x64 tiles while they give every CU a
+//    workgroup, else 64x64; split K when the 64x64 grid still leaves CUs idle
+//    (FP32/FP64) or gives them fewer than two workgroups (BF16), keeping every
+//    K slice at least 512 deep.
 struct Choice {
   int cfg, split;
 };
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
 constexpr int large_cfg(int mode) { return mode == 0 ? 0 : (mode == 3 ? 2 : 3); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
+constexpr int deep_cfg(int mode) { return split_mode(mode) ? 5 : 2; }
+constexpr int clamp_split(int split, int k, int min_depth) {
+  int max_by_depth = k / min_depth;
+  if (split > max_by_depth) split = max_by_depth;
+  if (split > 64) split = 64;
+  return split < 1 ? 1 : split;
+}
 constexpr Choice heuristic(int mode, int m, int n, int k) {
   int cfg = large_cfg(mode);
-  if (tiles_of(mode, cfg, m, n) < 2 * kCUs) {
-    cfg = medium_cfg(mode);
-    if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;  // 64x64
+  if (tiles_of(mode, cfg, m, n) >= 2 * kCUs) return Choice{cfg, 1};
+  const int dc = deep_cfg(mode);
+  const int dtiles = tiles_of(mode, dc, m, n);
+  if (k >= 16384 && dtiles < kCUs) {
+    const int per_cu = split_mode(mode) ? 1 : 2;
+    return Choice{dc, clamp_split(cdiv(per_cu * kCUs, dtiles), k, 2 * kDims[mode][dc].bk)};
   }
-  int tiles = tiles_of(mode, cfg, m, n);
+  cfg = medium_cfg(mode);
+  if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;  // 64x64
+  const int tiles = tiles_of(mode, cfg, m, n);
+  const int want = split_mode(mode) ? 2 * kCUs : kCUs;
   int split = 1;
-  if (tiles < kCUs && k >= 1024) {
-    split = cdiv(4 * kCUs, tiles);
-    int max_by_depth = k / 512;
-    if (split > max_by_depth) split = max_by_depth;
-    if (split > 64) split = 64;
-    if (split < 1) split = 1;
-  }
+  if (tiles < want && k >= 1024) split = clamp_split(cdiv(4 * kCUs, tiles), k, 512);
   return Choice{cfg, split};
 }
 

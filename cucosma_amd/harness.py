@@ -1,0 +1,309 @@
+"""Benchmark harness for every workload configuration of the hot path.
+
+The reference's harness runs each implementation 10 times untimed and 100
+times timed with a cache flush after every run (Kernel/cuda/src/benchmark.h:30-70,
+Kernel/hip/src/benchmark.h:39-91, Cacheflusher.h:18-49) and writes
+"<M>x<N>x<K><name>.csv" files with an "Implementation,ms" header.  This
+module keeps that protocol (through the native ``cosma_benchmark`` entry
+point) and that CSV format, and adds what the reference never reported:
+TFLOP/s as a share of the MI355X peak for the arithmetic used, HBM GB/s for
+the HBM-bound flat shape, and normwise error against an FP64 product.
+
+Configurations:
+  plain    1024^3, alpha 1, beta 0.5, A/B/C uniform[-1,1] from one MT19937(0) stream
+  square   8192^3, A/B uniform[-1,1] seed 1, beta 0: fp32, bf16x3, bf16x4
+  flat     256 x 256 x 1,048,576, seed 1, beta 0, split-K 8..64: fp32, bf16x3, bf16x4
+  dgemm    32768x1024x1024 and 1024x65536x1024, normal(0,1) seed 2 with 1% of
+           entries x1e6, C normal(0,1), alpha = beta = 1
+  grouped  512 problems, M/N/K log-uniform in [32, 2048] rounded to 32, seed 3:
+           one hipGraph replay of the persistent grouped launch vs one launch per problem
+  autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
+
+Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2516.6 / 3, "bf16x4": 2516.6 / 4}
+HBM_PEAK = 8000.0  # GB/s
+FLUSH_BYTES = 512 << 20
+
+
+def _desc(cc, mode, A, B, Cm, alpha, beta, split_k=0, tile_cfg=-1, group_m=0):
+    from ._lib import MODES, GemmDesc
+    from .gemm import _ld, _stream_ptr
+
+    d = GemmDesc()
+    cc.lib.cosma_gemm_desc_init(C.byref(d))
+    d.mode = MODES[mode]
+    d.m, d.k = A.shape
+    d.n = B.shape[1]
+    d.a, d.lda = A.data_ptr(), _ld(A, "A")
+    d.b, d.ldb = B.data_ptr(), _ld(B, "B")
+    d.c, d.ldc = Cm.data_ptr(), _ld(Cm, "C")
+    d.alpha, d.beta = alpha, beta
+    d.split_k, d.tile_cfg, d.group_m = split_k, tile_cfg, group_m
+    d.stream = _stream_ptr(None)
+    return d
+
+
+def bench_gemm(cc, mode, A, B, Cm, alpha=1.0, beta=0.0, runs=100, warmup=10, flush=True, **kw):
+    """Reference protocol through cosma_benchmark; returns per-run ms."""
+    d = _desc(cc, mode, A, B, Cm, alpha, beta, **kw)
+    ms = (C.c_float * max(runs, 1))()
+    cc.check(cc.lib.cosma_benchmark(C.byref(d), warmup, runs, FLUSH_BYTES if flush else 0, ms),
+             "cosma_benchmark")
+    return np.array(ms[:runs], dtype=np.float64)
+
+
+def _summary(name, mode, M, N, K, ms, extra=None):
+    med = float(np.median(ms))
+    flops = 2.0 * M * N * K
+    out = dict(config=name, mode=mode, M=M, N=N, K=K, runs=len(ms), ms_median=med,
+               ms_mean=float(ms.mean()), ms_min=float(ms.min()),
+               tflops=flops / med / 1e9, pct_peak=100 * flops / med / 1e9 / PEAK[mode])
+    if extra:
+        out.update(extra)
+    return out
+
+
+def _write_csv(csv_dir, M, N, K, name, ms):
+    if not csv_dir:
+        return
+    os.makedirs(csv_dir, exist_ok=True)
+    with open(os.path.join(csv_dir, f"{M}x{N}x{K}{name}.csv"), "w") as f:
+        f.write("Implementation,ms\n")
+        for v in ms:
+            f.write(f"{name},{v:.6g}\n")
+
+
+def _err(cc, A, B, Cm, R):
+    e = cc.normwise_error(A, B, Cm, R)
+    return dict(normwise_err=e["normwise"], max_rel_err=e["max_rel"], max_abs_err=e["max_abs"])
+
+
+def cfg_plain(cc, torch, args, emit):
+    M = N = K = 1024
+    rs = np.random.RandomState(0)
+    A = torch.from_numpy(rs.uniform(-1, 1, (M, K)).astype(np.float32)).cuda()
+    B = torch.from_numpy(rs.uniform(-1, 1, (K, N)).astype(np.float32)).cuda()
+    C0 = torch.from_numpy(rs.uniform(-1, 1, (M, N)).astype(np.float32)).cuda()
+    R = cc.fp64_product(A, B, C0, 1.0, 0.5)
+    for mode in ("fp32", "bf16x3"):
+        Cm = C0.clone()
+        cc.gemm(A, B, Cm, alpha=1.0, beta=0.5, mode=mode)
+        err = _err(cc, A, B, Cm, R)
+        ms = bench_gemm(cc, mode, A, B, C0.clone(), 1.0, 0.5, args.runs, args.warmup, not args.no_flush)
+        _write_csv(args.csv_dir, M, N, K, "cosma_" + mode, ms)
+        emit(_summary("plain", mode, M, N, K, ms, err))
+
+
+def cfg_square(cc, torch, args, emit):
+    M = N = K = 8192
+    rs = np.random.RandomState(1)
+    A = torch.from_numpy(rs.uniform(-1, 1, (M, K)).astype(np.float32)).cuda()
+    B = torch.from_numpy(rs.uniform(-1, 1, (K, N)).astype(np.float32)).cuda()
+    R = cc.fp64_product(A, B)
+    for mode in ("fp32", "bf16x3", "bf16x4"):
+        Cm = torch.empty((M, N), device="cuda")
+        cc.gemm(A, B, Cm, mode=mode)
+        err = _err(cc, A, B, Cm, R)
+        ms = bench_gemm(cc, mode, A, B, Cm, 1.0, 0.0, args.runs, args.warmup, not args.no_flush)
+        _write_csv(args.csv_dir, M, N, K, "cosma_" + mode, ms)
+        emit(_summary("square", mode, M, N, K, ms, err))
+    del R
+
+
+def cfg_flat(cc, torch, args, emit):
+    M, N, K = 256, 256, 1 << 20
+    rs = np.random.RandomState(1)
+    A = torch.from_numpy(rs.uniform(-1, 1, (M, K)).astype(np.float32)).cuda()
+    B = torch.from_numpy(rs.uniform(-1, 1, (K, N)).astype(np.float32)).cuda()
+    R = cc.fp64_product(A, B)
+    alg_bytes = (M * K + K * N + M * N) * 4
+    for mode in ("fp32", "bf16x3", "bf16x4"):
+        for split in (8, 16, 32, 64, 128):
+            Cm = torch.empty((M, N), device="cuda")
+            cc.gemm(A, B, Cm, mode=mode, split_k=split, tile_cfg=1)
+            err = _err(cc, A, B, Cm, R)
+            ms = bench_gemm(cc, mode, A, B, Cm, 1.0, 0.0, args.runs, args.warmup, not args.no_flush,
+                            split_k=split, tile_cfg=1)
+            med = float(np.median(ms))
+            gbs = alg_bytes / (med * 1e-3) / 1e9
+            _write_csv(args.csv_dir, M, N, K, f"cosma_{mode}_sk{split}", ms)
+            emit(_summary("flat", mode, M, N, K, ms, dict(err, split_k=split, tile="64x64",
+                                                           hbm_gbs=gbs, pct_hbm=100 * gbs / HBM_PEAK)))
+
+
+def cfg_dgemm(cc, torch, args, emit):
+    for (M, N, K) in ((32768, 1024, 1024), (1024, 65536, 1024)):
+        rs = np.random.RandomState(2)
+        A = rs.normal(0, 1, (M, K))
+        A[rs.random_sample((M, K)) < 0.01] *= 1e6
+        B = rs.normal(0, 1, (K, N))
+        B[rs.random_sample((K, N)) < 0.01] *= 1e6
+        C0 = rs.normal(0, 1, (M, N))
+        Ad, Bd, Cd = (torch.from_numpy(x).cuda() for x in (A, B, C0))
+        ms = bench_gemm(cc, "fp64", Ad, Bd, Cd, 1.0, 1.0, args.runs, args.warmup, not args.no_flush)
+        _write_csv(args.csv_dir, M, N, K, "cosma_dgemm", ms)
+        emit(_summary("dgemm", "fp64", M, N, K, ms))
+
+
+def grouped_sizes(count=512, seed=3, lo=32, hi=2048):
+    """M, N, K each log-uniform in [lo, hi], rounded to a multiple of 32."""
+    rs = np.random.RandomState(seed)
+    d = np.exp(rs.uniform(np.log(lo), np.log(hi), (count, 3)))
+    d = np.clip(np.round(d / 32) * 32, lo, hi).astype(int)
+    return [tuple(int(x) for x in row) for row in d]
+
+
+def cfg_grouped(cc, torch, args, emit):
+    sizes = grouped_sizes()
+    flops = sum(2.0 * m * n * k for m, n, k in sizes)
+    for mode in ("fp32", "bf16x3"):
+        g = torch.Generator(device="cuda").manual_seed(3)
+        As = [torch.rand((m, k), device="cuda", generator=g) * 2 - 1 for m, n, k in sizes]
+        Bs = [torch.rand((k, n), device="cuda", generator=g) * 2 - 1 for m, n, k in sizes]
+        Cs = [torch.empty((m, n), device="cuda") for m, n, k in sizes]
+        # Size buckets (largest output side): the tile per bucket is chosen by
+        # timing every combination of the candidate tiles.
+        edges = (256, 1024)
+        bucket = [0 if max(m, n) < edges[0] else (1 if max(m, n) < edges[1] else 2)
+                  for m, n, k in sizes]
+        cand = (1, 0) if mode == "fp32" else (1, 0, 2)
+        best = None
+        tried = []
+        for c0 in cand:
+            for c1 in cand:
+                for c2 in cand:
+                    cfgs = [(c0, c1, c2)[b] for b in bucket]
+                    plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=cfgs)
+                    plan.run_graph()
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    for _ in range(5):
+                        plan.run_graph()
+                    torch.cuda.synchronize()
+                    t = (time.perf_counter() - t0) / 5
+                    tried.append(((c0, c1, c2), t))
+                    if best is None or t < best[1]:
+                        best = ((c0, c1, c2), t)
+                    plan.close()
+        cfgs = [best[0][b] for b in bucket]
+        plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=cfgs)
+        plan.run_graph()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(args.runs):
+            plan.run_graph()
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms_graph = ev[0].elapsed_time(ev[1]) / args.runs
+        # one launch per problem (library heuristic per shape)
+        for A, B, Cm in zip(As, Bs, Cs):
+            cc.gemm(A, B, Cm, mode=mode)
+        torch.cuda.synchronize()
+        reps = max(1, args.runs // 10)
+        ev[0].record()
+        for _ in range(reps):
+            for A, B, Cm in zip(As, Bs, Cs):
+                cc.gemm(A, B, Cm, mode=mode)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms_loop = ev[0].elapsed_time(ev[1]) / reps
+        info = plan.info()
+        emit(dict(config="grouped", mode=mode, problems=len(sizes), gflop=flops / 1e9,
+                  bucket_edges=list(edges), bucket_tile_cfgs=list(best[0]), buckets=info["buckets"],
+                  ms_graph=ms_graph, tflops_graph=flops / ms_graph / 1e9,
+                  ms_per_problem_launches=ms_loop, tflops_per_problem_launches=flops / ms_loop / 1e9,
+                  speedup=ms_loop / ms_graph,
+                  pct_peak=100 * flops / ms_graph / 1e9 / PEAK[mode]))
+        plan.close()
+        del As, Bs, Cs
+
+
+def cfg_autograd(cc, torch, args, emit):
+    g = torch.Generator(device="cpu").manual_seed(4)
+    X = (torch.rand(16384, 4096, generator=g) * 2 - 1).cuda().requires_grad_()
+    W = (torch.randn(4096, 4096, generator=g) / 64).cuda().requires_grad_()
+    dY = torch.randn(16384, 4096, generator=g).cuda()
+    flops = 2.0 * 16384 * 4096 * 4096
+    for mode in ("bf16x3", "fp32"):
+        for _ in range(3):
+            Y = cc.matmul(X, W, mode)
+            Y.backward(dY)
+        torch.cuda.synchronize()
+        reps = max(3, args.runs // 10)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        fwd = bwd = 0.0
+        for _ in range(reps):
+            X.grad = W.grad = None
+            ev[0].record()
+            Y = cc.matmul(X, W, mode)
+            ev[1].record()
+            Y.backward(dY)
+            ev[2].record()
+            torch.cuda.synchronize()
+            fwd += ev[0].elapsed_time(ev[1])
+            bwd += ev[1].elapsed_time(ev[2])
+        fwd /= reps
+        bwd /= reps
+        emit(dict(config="autograd", mode=mode, M=16384, N=4096, K=4096, ms_forward=fwd,
+                  ms_backward=bwd, tflops_forward=flops / fwd / 1e9,
+                  tflops_backward=2 * flops / bwd / 1e9,
+                  pct_peak_step=100 * 3 * flops / (fwd + bwd) / 1e9 / PEAK[mode]))
+
+
+CONFIGS = dict(plain=cfg_plain, square=cfg_square, flat=cfg_flat, dgemm=cfg_dgemm,
+               grouped=cfg_grouped, autograd=cfg_autograd)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="cosma benchmark harness")
+    ap.add_argument("--config", default="all", choices=["all", *CONFIGS])
+    ap.add_argument("--runs", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-flush", action="store_true", help="skip the cache flush between runs")
+    ap.add_argument("--csv-dir", help="write <M>x<N>x<K><name>.csv files (reference format)")
+    ap.add_argument("--tune-db", help="apply this autotuner DB before running")
+    ap.add_argument("--out", help="append JSON lines here")
+    args = ap.parse_args(argv)
+
+    import torch
+
+    import cucosma_amd as cc
+
+    if args.tune_db:
+        from .autotune import TuneDB
+
+        TuneDB(args.tune_db).apply()
+    out = open(args.out, "a") if args.out else None
+
+    def emit(rec):
+        line = json.dumps(rec)
+        print(line, flush=True)
+        if out:
+            out.write(line + "\n")
+            out.flush()
+
+    names = list(CONFIGS) if args.config == "all" else [args.config]
+    for name in names:
+        print(f"[harness] {name}", file=sys.stderr, flush=True)
+        CONFIGS[name](cc, torch, args, emit)
+        torch.cuda.empty_cache()
+    if out:
+        out.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
