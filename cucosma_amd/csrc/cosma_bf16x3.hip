@@ -1,12 +1,10 @@
 // cosma_bf16x3.hip -- split-precision SGEMM (bf16 hi/lo, three products on
 // v_mfma_f32_16x16x32_bf16, FP32 accumulation): tile configurations.
+// Tile<mode, BM, BN, BK, WM, WN, PIPE, OCC> as in cosma_f32.hip.
 #include "cosma_launch.hpp"
 
 namespace cosma {
 
-// cfg 0: 128x128 workgroup tile, 4 waves of 64x64
-// cfg 1: 64x64, 4 waves of 32x32
-// cfg 2: 256x128, 8 waves of 64x64
 using B3C0 = Tile<kBF16x3, 128, 128, 32, 64, 64>;
 using B3C1 = Tile<kBF16x3, 64, 64, 32, 32, 32>;
 using B3C2 = Tile<kBF16x3, 256, 128, 32, 64, 64>;

@@ -1,17 +1,16 @@
 // cosma_bf16x4.hip -- split-precision SGEMM with four products (bf16 hi/lo,
 // hi*hi + hi*lo + lo*hi + lo*lo on v_mfma_f32_16x16x32_bf16, FP32
-// accumulation): tile configurations, the same as cosma_bf16x3.hip.
+// accumulation): the tiles of cosma_bf16x3.hip; cfg 3 places the next-stage
+// copy between the MFMAs (PIPE 1), which leaves room for the per-k-step
+// temporaries without spilling.
 #include "cosma_launch.hpp"
 
 namespace cosma {
 
-// cfg 0: 128x128 workgroup tile, 4 waves of 64x64
-// cfg 1: 64x64, 4 waves of 32x32
-// cfg 2: 256x128, 8 waves of 64x64
 using B4C0 = Tile<kBF16x4, 128, 128, 32, 64, 64>;
 using B4C1 = Tile<kBF16x4, 64, 64, 32, 32, 32>;
 using B4C2 = Tile<kBF16x4, 256, 128, 32, 64, 64>;
-using B4C3 = Tile<kBF16x4, 256, 256, 32, 128, 64>;
+using B4C3 = Tile<kBF16x4, 256, 256, 32, 128, 64, 1>;
 using B4C4 = Tile<kBF16x4, 256, 128, 32, 128, 64>;
 using B4C5 = Tile<kBF16x4, 128, 128, 64, 64, 64>;
 

@@ -1,20 +1,20 @@
 // cosma_f32.hip -- FP32 GEMM on v_mfma_f32_16x16x4_f32: tile configurations.
+// Tile<mode, BM, BN, BK, WM, WN, PIPE, OCC>: workgroup tile BM x BN with a K
+// chunk of BK, waves of WM x WN, next-stage copy placement PIPE (see
+// compute_stage), minimum waves per SIMD OCC.
 #include "cosma_launch.hpp"
 
 namespace cosma {
 
-// cfg 0: 128x128 workgroup tile, 4 waves of 64x64 (the default for large shapes)
-// cfg 1: 64x64, 4 waves of 32x32 (small shapes, split-K slices)
-// cfg 2: 128x64, 4 waves of 64x32
-// cfg 3: 256x128, 8 waves of 64x64
-using F32C0 = Tile<kF32, 128, 128, 32, 64, 64>;
-using F32C1 = Tile<kF32, 64, 64, 32, 32, 32>;
-using F32C2 = Tile<kF32, 128, 64, 32, 64, 32>;
-using F32C3 = Tile<kF32, 256, 128, 32, 64, 64>;
-using F32C4 = Tile<kF32, 128, 128, 64, 64, 64>;
+using F32C0 = Tile<kF32, 128, 128, 32, 64, 64, 1>;
+using F32C1 = Tile<kF32, 64, 64, 32, 32, 32, 1>;
+using F32C2 = Tile<kF32, 128, 64, 32, 64, 32, 1>;
+using F32C3 = Tile<kF32, 256, 128, 32, 64, 64, 1>;
+using F32C4 = Tile<kF32, 128, 128, 64, 64, 64, 1>;
+using F32C5 = Tile<kF32, 256, 128, 32, 64, 64, 3>;
 
-extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info(), F32C4::info()};
-extern const int kF32NumTiles = 5;
+extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info(), F32C4::info(), F32C5::info()};
+extern const int kF32NumTiles = 6;
 
 hipError_t run_gemm_f32(const Problem& p) {
   switch (p.cfg) {
@@ -23,6 +23,7 @@ hipError_t run_gemm_f32(const Problem& p) {
     case 2: return F32C2::run(p);
     case 3: return F32C3::run(p);
     case 4: return F32C4::run(p);
+    case 5: return F32C5::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -35,6 +36,7 @@ hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int
     case 2: return F32C2::grouped(table, count, total, blocks, s);
     case 3: return F32C3::grouped(table, count, total, blocks, s);
     case 4: return F32C4::grouped(table, count, total, blocks, s);
+    case 5: return F32C5::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -46,6 +48,7 @@ int grouped_occupancy_f32(int cfg) {
     case 2: return F32C2::grouped_occupancy();
     case 3: return F32C3::grouped_occupancy();
     case 4: return F32C4::grouped_occupancy();
+    case 5: return F32C5::grouped_occupancy();
     default: return 1;
   }
 }
@@ -57,6 +60,7 @@ int gemm_occupancy_f32(int cfg, int ta, int tb, bool edge) {
     case 2: return F32C2::occupancy(ta, tb, edge);
     case 3: return F32C3::occupancy(ta, tb, edge);
     case 4: return F32C4::occupancy(ta, tb, edge);
+    case 5: return F32C5::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

@@ -1,19 +1,19 @@
 // cosma_f64.hip -- FP64 GEMM on v_mfma_f64_16x16x4_f64: tile configurations.
+// Tile<mode, BM, BN, BK, WM, WN, PIPE, OCC> as in cosma_f32.hip.
 #include "cosma_launch.hpp"
 
 namespace cosma {
 
-// cfg 0: 128x128 workgroup tile, 4 waves of 64x64
-// cfg 1: 64x64, 4 waves of 32x32
-// cfg 2: 128x64, 4 waves of 64x32
 using F64C0 = Tile<kF64, 128, 128, 16, 64, 64>;
 using F64C1 = Tile<kF64, 64, 64, 16, 32, 32>;
 using F64C2 = Tile<kF64, 128, 64, 16, 64, 32>;
-using F64C3 = Tile<kF64, 128, 128, 16, 64, 32>;
+using F64C3 = Tile<kF64, 128, 128, 16, 64, 32, 0, 4>;
 using F64C4 = Tile<kF64, 64, 64, 32, 32, 32>;
+using F64C5 = Tile<kF64, 128, 128, 32, 64, 32>;
+using F64C6 = Tile<kF64, 256, 128, 16, 64, 64>;
 
-extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info(), F64C3::info(), F64C4::info()};
-extern const int kF64NumTiles = 5;
+extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info(), F64C3::info(), F64C4::info(), F64C5::info(), F64C6::info()};
+extern const int kF64NumTiles = 7;
 
 hipError_t run_gemm_f64(const Problem& p) {
   switch (p.cfg) {
@@ -22,6 +22,8 @@ hipError_t run_gemm_f64(const Problem& p) {
     case 2: return F64C2::run(p);
     case 3: return F64C3::run(p);
     case 4: return F64C4::run(p);
+    case 5: return F64C5::run(p);
+    case 6: return F64C6::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -34,6 +36,8 @@ hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int
     case 2: return F64C2::grouped(table, count, total, blocks, s);
     case 3: return F64C3::grouped(table, count, total, blocks, s);
     case 4: return F64C4::grouped(table, count, total, blocks, s);
+    case 5: return F64C5::grouped(table, count, total, blocks, s);
+    case 6: return F64C6::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -45,6 +49,8 @@ int grouped_occupancy_f64(int cfg) {
     case 2: return F64C2::grouped_occupancy();
     case 3: return F64C3::grouped_occupancy();
     case 4: return F64C4::grouped_occupancy();
+    case 5: return F64C5::grouped_occupancy();
+    case 6: return F64C6::grouped_occupancy();
     default: return 1;
   }
 }
@@ -56,6 +62,8 @@ int gemm_occupancy_f64(int cfg, int ta, int tb, bool edge) {
     case 2: return F64C2::occupancy(ta, tb, edge);
     case 3: return F64C3::occupancy(ta, tb, edge);
     case 4: return F64C4::occupancy(ta, tb, edge);
+    case 5: return F64C5::occupancy(ta, tb, edge);
+    case 6: return F64C6::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

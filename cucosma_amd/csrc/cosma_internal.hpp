@@ -35,6 +35,7 @@ struct TileInfo {
   int BM, BN, BK, WM, WN;
   int threads;
   int lds_bytes;
+  int pipe;  // 0: next-stage copy after the MFMAs, 1: between them
 };
 
 // One problem of a grouped launch, as the grouped kernel reads it from

@@ -197,39 +197,50 @@ struct TileLoader {
     return out;
   }
 
+  // A thread's units are UNIT_STEP rows (KMAJ) or columns (otherwise) apart,
+  // a compile-time stride, so one 64-bit pointer to its first unit plus
+  // scalar offsets (multiples of ld) address all of them.
+  static constexpr int VPR = BK / GVEC;  // KMAJ: vectors per row; otherwise k blocks per tile
+  static constexpr int UNIT_STEP = KMAJ ? NT / VPR : (NT / VPR) * GVEC;
+  static_assert(NT % VPR == 0, "unit stride must be uniform across the workgroup");
+  const G* p0;
+  int r0_;  // KMAJ: row of the first unit; otherwise its first column
+  int k0_;  // first k of every unit of this thread
+  int64_t ld_;
+
   // base: operand pointer; ld: its leading dimension; r0: first tile row;
-  // rmax: rows in the problem (M or N); k0: first k of the chunk; kend: end of
-  // the valid k range of this split.
-  __device__ __forceinline__ void load(const G* __restrict__ base, int64_t ld, int r0, int rmax,
-                                       int k0, int kend, int tid, bool vec_ok) {
+  // k0: first k of the tile's K slice.
+  __device__ __forceinline__ void init(const G* __restrict__ base, int64_t ld, int r0, int k0, int tid) {
+    ld_ = ld;
+    const int a = tid / VPR;
+    const int b = tid - a * VPR;
     if constexpr (KMAJ) {
-      constexpr int VPR = BK / GVEC;  // vectors per row
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int idx = tid + i * NT;
-        if (!active(idx)) continue;
-        const int r = idx / VPR;
-        const int kv = idx - r * VPR;
-        const int gr = r0 + r;
-        const int gk = k0 + kv * GVEC;
-        const G* p = base + (int64_t)gr * ld + gk;
-        v[i] = load_vec(p, gr < rmax, kend - gk, vec_ok);
-      }
+      r0_ = r0 + a;
+      k0_ = k0 + b * GVEC;
+      p0 = base + (int64_t)r0_ * ld + k0_;
     } else {
-      constexpr int KB = BK / GVEC;
+      r0_ = r0 + a * GVEC;
+      k0_ = k0 + b * GVEC;
+      p0 = base + (int64_t)k0_ * ld + r0_;
+    }
+  }
+
+  // Loads the chunk that starts kofs past init()'s k0.  rmax: rows (KMAJ) or
+  // columns of the operand; kend: end of the valid k range of this split.
+  __device__ __forceinline__ void load(int kofs, int rmax, int kend, int tid, bool vec_ok) {
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int idx = tid + i * NT;
-        if (!active(idx)) continue;
-        const int rb = idx / KB;
-        const int kb = idx - rb * KB;
-        const int gr = r0 + rb * GVEC;
+    for (int i = 0; i < PER; ++i) {
+      if (!active(tid + i * NT)) continue;
+      const int rr = r0_ + i * UNIT_STEP;
+      const int gk = k0_ + kofs;
+      if constexpr (KMAJ) {
+        const G* p = p0 + (int64_t)(i * UNIT_STEP) * ld_ + kofs;
+        v[i] = load_vec(p, rr < rmax, kend - gk, vec_ok);
+      } else {
+        const G* p = p0 + (int64_t)kofs * ld_ + i * UNIT_STEP;
 #pragma unroll
-        for (int u = 0; u < GVEC; ++u) {
-          const int gk = k0 + kb * GVEC + u;
-          const G* p = base + (int64_t)gk * ld + gr;
-          v[i * GVEC + u] = load_vec(p, gk < kend, rmax - gr, vec_ok);
-        }
+        for (int u = 0; u < GVEC; ++u)
+          v[i * GVEC + u] = load_vec(p + (int64_t)u * ld_, gk + u < kend, rmax - rr, vec_ok);
       }
     }
   }
@@ -297,29 +308,6 @@ __device__ __forceinline__ int acc_row(int g, int r) {
   return MODE == kF64 ? g + 4 * r : 4 * g + r;
 }
 
-// Epilogue of one 16x16 sub-tile (rows sub_row0.., column col of this lane):
-// split == 1 writes alpha*acc (+ beta*C when beta != 0; C is not read when
-// beta == 0); split > 1 writes the raw partial to ws[z][row][col].
-template <int MODE, bool EDGE>
-__device__ __forceinline__ void epilogue_store(const GemmParams<MODE>& p, int z, int sub_row0, int g,
-                                               int col, const typename Traits<MODE>::Acc& acc) {
-  using E = typename Traits<MODE>::E;
-  if (EDGE && col >= p.N) return;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = sub_row0 + acc_row<MODE>(g, r);
-    if (EDGE && row >= p.M) continue;
-    if (p.split == 1) {
-      E v = p.alpha * acc[r];
-      E* cp = p.C + (int64_t)row * p.ldc + col;
-      if (p.beta != E(0)) v = fma_e(p.beta, *cp, v);
-      *cp = v;
-    } else {
-      p.ws[((int64_t)z * p.M + row) * p.N + col] = acc[r];
-    }
-  }
-}
-
 template <int MODE, int BM, int BN, int BK, int WM, int WN>
 struct TileShape {
   static constexpr int WAVES_M = BM / WM;
@@ -336,106 +324,185 @@ struct TileShape {
 };
 
 // MFMA sequence over one LDS stage for one wave's TM x TN sub-tiles.
-template <int MODE, int BM, int BN, int BK, int WM, int WN>
+//
+// PIPE selects how the copy of the next stage (mid(): split + LDS writes of
+// the registers loaded at the top of the iteration) is placed:
+//   0  all fragments of a k-round are read first and mid() runs after every
+//      MFMA has been issued -- the longest time for the global loads to land,
+//      and the widest distance between MFMAs on one accumulator;
+//   1  B fragments are read per k-round, A fragments per sub-tile row, and
+//      mid() runs between the two halves of the rows of the last k-round, so
+//      its VALU and LDS work overlaps MFMAs still queued;
+//   2  as 1, with mid() after three quarters of the rows (more time for the
+//      global loads to land);
+//   3  as 1, with mid() after the first quarter of the rows; the caller
+//      double-buffers the loader registers so mid()'s data arrived an
+//      iteration earlier (gemm_tile).
+// Which is faster depends on the tile (register headroom, MFMA latency of the
+// type); each tile configuration fixes its choice from measurement.
+template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, typename Mid>
 __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __restrict__ st,
                                               typename Traits<MODE>::Acc (&acc)[WM / 16][WN / 16],
-                                              int wm0, int wn0, int lane) {
+                                              int wm0, int wn0, int lane, Mid&& mid) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   using T = Traits<MODE>;
   using Frag = typename T::Frag;
+  using Acc = typename T::Acc;
   constexpr int TM = Sh::TM, TN = Sh::TN, LDK = Sh::LDK;
+  constexpr int NQ = BK / Sh::KR;
+  constexpr int kMidRow = PIPE == 1 ? TM / 2 : (PIPE == 2 ? (3 * TM) / 4 : TM / 4);  // row before which mid() runs
   const int g = lane >> 4;
   const int r16 = lane & 15;
   const typename T::S* sa = st;
   const typename T::S* sb = st + BM * LDK;
+
+  // One sub-tile row of the split modes: smallest terms first, then the
+  // leading hi*hi term, the product index outermost so TN independent MFMAs
+  // separate two updates of one accumulator.  kBF16x3 chains the three
+  // products into the running accumulator; kBF16x4 sums its four products of
+  // this k-step in a zeroed temporary and adds that with one round-to-nearest
+  // VALU add, so the large running sum is rounded once per k-step.
+  // (Generic lambdas: their bodies are instantiated only for the modes that
+  // call them.)
+  auto split_row = [&](auto& row, const auto& ah, const auto& al, const auto& bh, const auto& bl) {
+    if constexpr (MODE == kBF16x4) {
+      Acc t[TN];
 #pragma unroll
-  for (int q = 0; q < BK / Sh::KR; ++q) {
+      for (int j = 0; j < TN; ++j) t[j] = zero_acc<MODE>();
+#pragma unroll
+      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j], t[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], t[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], t[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], t[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) row[j][r] += t[j][r];
+    } else {
+#pragma unroll
+      for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], row[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], row[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], row[j], 0, 0, 0);
+    }
+  };
+  auto mfma1 = [&](Acc& c, auto a, auto b) {
+    if constexpr (MODE == kF32)
+      c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    else
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  };
+
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
     const int kof = q * Sh::KR + g * T::KPL;
     if constexpr (is_split(MODE)) {
       const typename T::S* sal = st + Sh::PLANE;
       const typename T::S* sbl = sal + BM * LDK;
-      Frag ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int off = (wm0 + i * 16 + r16) * LDK + kof;
-        ah[i] = *reinterpret_cast<const Frag*>(sa + off);
-        al[i] = *reinterpret_cast<const Frag*>(sal + off);
-      }
+      Frag bh[TN], bl[TN];
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         const int off = (wn0 + j * 16 + r16) * LDK + kof;
         bh[j] = *reinterpret_cast<const Frag*>(sb + off);
         bl[j] = *reinterpret_cast<const Frag*>(sbl + off);
       }
-      // Smallest terms first, then the leading hi*hi term.  kBF16x3 chains
-      // the three products into the running accumulator.  kBF16x4 sums its
-      // four products of this k-step in a zeroed temporary and adds that to
-      // the accumulator with one round-to-nearest VALU add, so the large
-      // running sum is rounded once per k-step instead of four times.
-      // Within one row of sub-tiles the product index is the outer loop, so
-      // TN independent MFMAs separate two that update the same accumulator.
+      if constexpr (PIPE == 0) {
+        Frag ah[TM], al[TM];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        if constexpr (MODE == kBF16x4) {
-          typename T::Acc t[TN];
+        for (int i = 0; i < TM; ++i) {
+          const int off = (wm0 + i * 16 + r16) * LDK + kof;
+          ah[i] = *reinterpret_cast<const Frag*>(sa + off);
+          al[i] = *reinterpret_cast<const Frag*>(sal + off);
+        }
 #pragma unroll
-          for (int j = 0; j < TN; ++j) t[j] = zero_acc<MODE>();
+        for (int i = 0; i < TM; ++i) split_row(acc[i], ah[i], al[i], bh, bl);
+      } else {
 #pragma unroll
-          for (int j = 0; j < TN; ++j)
-            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl[j], t[j], 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], t[j], 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], t[j], 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], t[j], 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[i][j][r] += t[j][r];
-        } else {
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < TM; ++i) {
+          if (q == NQ - 1 && i == kMidRow) mid();
+          const int off = (wm0 + i * 16 + r16) * LDK + kof;
+          const Frag ah = *reinterpret_cast<const Frag*>(sa + off);
+          const Frag al = *reinterpret_cast<const Frag*>(sal + off);
+          split_row(acc[i], ah, al, bh, bl);
         }
       }
     } else {
-      Frag a[TM], b[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-        a[i] = *reinterpret_cast<const Frag*>(sa + (wm0 + i * 16 + r16) * LDK + kof);
+      Frag b[TN];
 #pragma unroll
       for (int j = 0; j < TN; ++j)
         b[j] = *reinterpret_cast<const Frag*>(sb + (wn0 + j * 16 + r16) * LDK + kof);
-#pragma unroll
-      for (int s = 0; s < T::STEPS; ++s)
+      if constexpr (PIPE == 0) {
+        Frag a[TM];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
+          a[i] = *reinterpret_cast<const Frag*>(sa + (wm0 + i * 16 + r16) * LDK + kof);
 #pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            if constexpr (MODE == kF32)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
-            else
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
-          }
+        for (int s = 0; s < T::STEPS; ++s)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) mfma1(acc[i][j], a[i][s], b[j][s]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          if (q == NQ - 1 && i == kMidRow) mid();
+          const Frag a = *reinterpret_cast<const Frag*>(sa + (wm0 + i * 16 + r16) * LDK + kof);
+#pragma unroll
+          for (int s = 0; s < T::STEPS; ++s)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) mfma1(acc[i][j], a[s], b[j][s]);
+        }
+      }
     }
   }
+  if constexpr (PIPE == 0) mid();
+}
+
+// Epilogue of the whole wave tile.  split == 1: C = alpha*acc, plus beta*C
+// when beta != 0 (C is not read when beta == 0); split > 1: the raw partial
+// goes to ws[z] (scale 1, exact).  Destination, leading dimension, scale and
+// whether to read C are uniform and chosen once; one loop serves all cases
+// (separate loops per case made the compiler keep extra copies of the
+// accumulators, raising the kernel's register count).  Every address is a
+// per-lane base pointer plus a uniform (16 i + row) * ld offset.
+template <int MODE, bool EDGE, int TM, int TN>
+__device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int row0, int col0,
+                                         int lane, const typename Traits<MODE>::Acc (&acc)[TM][TN]) {
+  using E = typename Traits<MODE>::E;
+  const int g = lane >> 4;
+  const int lrow = row0 + (MODE == kF64 ? g : 4 * g);  // row of element 0 of this lane
+  const int lcol = col0 + (lane & 15);
+  const bool partial = p.split > 1;
+  const int64_t ld = partial ? (int64_t)p.N : p.ldc;
+  E* base = partial ? p.ws + ((int64_t)z * p.M + lrow) * p.N + lcol : p.C + (int64_t)lrow * p.ldc + lcol;
+  const E scale = partial ? E(1) : p.alpha;
+  const bool read_c = !partial && p.beta != E(0);
+  const E beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ro = i * 16 + (MODE == kF64 ? 4 * r : r);  // row offset of element r from lrow
+      E* q = base + (int64_t)ro * ld;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        if (EDGE && (lrow + ro >= p.M || lcol + j * 16 >= p.N)) continue;
+        E out = scale * acc[i][j][r];
+        if (read_c) out = fma_e(beta, q[j * 16], out);
+        q[j * 16] = out;
+      }
+    }
 }
 
 // One output tile (tm, tn) of split z: the K-slice main loop and the
 // epilogue.  lds holds two stages.  Ends with every wave past a barrier, so
 // a caller may start another tile on the same LDS right away.
 // EDGE == false requires M % BM == 0, N % BN == 0 and K % BK == 0.
-template <int MODE, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, bool EDGE>
+template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, bool TA, bool TB, bool EDGE>
 __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int tn, int z,
                                           typename Traits<MODE>::S* lds) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
@@ -458,6 +525,8 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
 
   TileLoader<MODE, BM, BK, NT, !TA, EDGE> la;
   TileLoader<MODE, BN, BK, NT, TB, EDGE> lb;
+  la.init(p.A, p.lda, m0, kbeg, tid);
+  lb.init(p.B, p.ldb, n0, kbeg, tid);
 
   typename T::Acc acc[Sh::TM][Sh::TN];
 #pragma unroll
@@ -466,49 +535,84 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
     for (int j = 0; j < Sh::TN; ++j) acc[i][j] = zero_acc<MODE>();
 
   auto stage_ptr = [&](int s) { return lds + s * Sh::STAGE; };
-  auto store_stage = [&](int s) {
-    S* st = stage_ptr(s);
-    la.store(st, st + Sh::PLANE, tid);
-    lb.store(st + BM * LDK, st + Sh::PLANE + BM * LDK, tid);
+
+  auto store_to = [&](const auto& a, const auto& b, S* st) {
+    a.store(st, st + Sh::PLANE, tid);
+    b.store(st + BM * LDK, st + Sh::PLANE + BM * LDK, tid);
   };
 
-  if (nk > 0) {
-    la.load(p.A, p.lda, m0, p.M, kbeg, kend, tid, p.vec_a);
-    lb.load(p.B, p.ldb, n0, p.N, kbeg, kend, tid, p.vec_b);
-    store_stage(0);
-    __syncthreads();
-  }
-  for (int it = 0; it < nk; ++it) {
-    const int cur = it & 1;
-    const bool more = it + 1 < nk;
-    if (more) {
-      const int k0 = kbeg + (it + 1) * BK;
-      la.load(p.A, p.lda, m0, p.M, k0, kend, tid, p.vec_a);
-      lb.load(p.B, p.ldb, n0, p.N, k0, kend, tid, p.vec_b);
+  if constexpr (PIPE == 3) {
+    // Two register sets: chunk it+2 is loaded while chunk it is computed and
+    // chunk it+1 (loaded one iteration earlier) is written to LDS among the
+    // MFMAs, so that copy never waits on memory.  The loop is unrolled by two
+    // so each set is named statically.
+    TileLoader<MODE, BM, BK, NT, !TA, EDGE> la1 = la;
+    TileLoader<MODE, BN, BK, NT, TB, EDGE> lb1 = lb;
+    if (nk > 0) {
+      la.load(0, p.M, kend, tid, p.vec_a);
+      lb.load(0, p.N, kend, tid, p.vec_b);
+      store_to(la, lb, stage_ptr(0));
+      const int k1 = (nk > 1 ? 1 : 0) * BK;
+      la1.load(k1, p.M, kend, tid, p.vec_a);
+      lb1.load(k1, p.N, kend, tid, p.vec_b);
+      __syncthreads();
     }
-    compute_stage<MODE, BM, BN, BK, WM, WN>(stage_ptr(cur), acc, wm0, wn0, lane);
-    if (more) store_stage(cur ^ 1);
-    __syncthreads();
+    // One iteration: the set `ld` was written to LDS last iteration and is
+    // free to receive chunk it+2 (clamped to the last chunk: always in
+    // range); the set `st` holds chunk it+1.
+    auto step = [&](int it, auto& lda_, auto& ldb_, const auto& sta, const auto& stb) {
+      const int cur = it & 1;
+      const int kofs = min(it + 2, nk - 1) * BK;
+      lda_.load(kofs, p.M, kend, tid, p.vec_a);
+      ldb_.load(kofs, p.N, kend, tid, p.vec_b);
+      S* nxt = stage_ptr(cur ^ 1);
+      compute_stage<MODE, BM, BN, BK, WM, WN, PIPE>(stage_ptr(cur), acc, wm0, wn0, lane,
+                                                    [&]() { store_to(sta, stb, nxt); });
+      __syncthreads();
+    };
+    int it = 0;
+    for (; it + 1 < nk; it += 2) {
+      step(it, la, lb, la1, lb1);
+      step(it + 1, la1, lb1, la, lb);
+    }
+    if (it < nk) step(it, la, lb, la1, lb1);
+  } else {
+    if (nk > 0) {
+      la.load(0, p.M, kend, tid, p.vec_a);
+      lb.load(0, p.N, kend, tid, p.vec_b);
+      store_to(la, lb, stage_ptr(0));
+      __syncthreads();
+    }
+    // Each iteration loads chunk it+1 (the last one reloads its own chunk, in
+    // range and never read) and writes it to the other stage during or after
+    // the MFMAs of chunk it (PIPE); no branch splits the loop body.
+    for (int it = 0; it < nk; ++it) {
+      const int cur = it & 1;
+      const int kofs = (it + 1 < nk ? it + 1 : it) * BK;
+      la.load(kofs, p.M, kend, tid, p.vec_a);
+      lb.load(kofs, p.N, kend, tid, p.vec_b);
+      S* nxt = stage_ptr(cur ^ 1);
+      compute_stage<MODE, BM, BN, BK, WM, WN, PIPE>(stage_ptr(cur), acc, wm0, wn0, lane,
+                                                    [&]() { store_to(la, lb, nxt); });
+      __syncthreads();
+    }
   }
 
-  const int g = lane >> 4;
-  const int r16 = lane & 15;
-#pragma unroll
-  for (int i = 0; i < Sh::TM; ++i)
-#pragma unroll
-    for (int j = 0; j < Sh::TN; ++j)
-      epilogue_store<MODE, EDGE>(p, z, m0 + wm0 + i * 16, g, n0 + wn0 + j * 16 + r16, acc[i][j]);
+  epilogue<MODE, EDGE, Sh::TM, Sh::TN>(p, z, m0 + wm0, n0 + wn0, lane, acc);
 }
 
 // The GEMM kernel.  Grid: tiles_m * tiles_n * split workgroups (1-D).
-template <int MODE, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, bool EDGE>
-__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
+// OCC: minimum waves per SIMD the register allocation must allow (1: no
+// constraint beyond the workgroup size).
+template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, int OCC, bool TA, bool TB,
+          bool EDGE>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amdgpu_waves_per_eu(OCC)))
     gemm_kernel(GemmParams<MODE> p) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
   int tm, tn, z;
   map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
-  gemm_tile<MODE, BM, BN, BK, WM, WN, TA, TB, EDGE>(p, tm, tn, z, lds);
+  gemm_tile<MODE, BM, BN, BK, WM, WN, PIPE, TA, TB, EDGE>(p, tm, tn, z, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -520,8 +624,8 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 // bounds-checked edges).  Tiles are numbered problem by problem, row-major
 // within a problem, so concurrently running workgroups share operand panels.
 // ---------------------------------------------------------------------------
-template <int MODE, int BM, int BN, int BK, int WM, int WN>
-__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
+template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, int OCC>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amdgpu_waves_per_eu(OCC)))
     grouped_kernel(const GroupEntry<typename Traits<MODE>::G>* __restrict__ table, int count,
                    int total_tiles) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
@@ -557,7 +661,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     const int local = t - e.tile_start;
     const int tm = local / e.tiles_n;
     const int tn = local - tm * e.tiles_n;
-    gemm_tile<MODE, BM, BN, BK, WM, WN, false, false, true>(p, tm, tn, 0, lds);
+    gemm_tile<MODE, BM, BN, BK, WM, WN, PIPE, false, false, true>(p, tm, tn, 0, lds);
   }
 }
 

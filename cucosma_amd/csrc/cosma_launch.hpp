@@ -6,24 +6,24 @@
 
 namespace cosma {
 
-template <int MODE, int BM, int BN, int BK, int WM, int WN>
+template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE = 0, int OCC = 1>
 struct Tile {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   static constexpr TileInfo info() {
     return TileInfo{BM, BN, BK, WM, WN, Sh::NT,
-                    (int)(2 * Sh::STAGE * sizeof(typename Traits<MODE>::S))};
+                    (int)(2 * Sh::STAGE * sizeof(typename Traits<MODE>::S)), PIPE};
   }
 
   template <bool TA, bool TB, bool EDGE>
   static hipError_t go(const GemmParams<MODE>& gp, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL((gemm_kernel<MODE, BM, BN, BK, WM, WN, TA, TB, EDGE>), dim3(nblk),
+    hipLaunchKernelGGL((gemm_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC, TA, TB, EDGE>), dim3(nblk),
                        dim3(Sh::NT), 0, s, gp);
     return hipGetLastError();
   }
 
   static hipError_t grouped(const void* table, int count, int total, int blocks, hipStream_t s) {
     using G = typename Traits<MODE>::G;
-    hipLaunchKernelGGL((grouped_kernel<MODE, BM, BN, BK, WM, WN>), dim3(blocks), dim3(Sh::NT), 0, s,
+    hipLaunchKernelGGL((grouped_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC>), dim3(blocks), dim3(Sh::NT), 0, s,
                        static_cast<const GroupEntry<G>*>(table), count, total);
     return hipGetLastError();
   }
@@ -31,7 +31,7 @@ struct Tile {
   static int grouped_occupancy() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&grouped_kernel<MODE, BM, BN, BK, WM, WN>), Sh::NT,
+            &n, reinterpret_cast<const void*>(&grouped_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC>), Sh::NT,
             0) != hipSuccess)
       return 1;
     return n > 0 ? n : 1;
@@ -42,7 +42,7 @@ struct Tile {
   static int occ() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&gemm_kernel<MODE, BM, BN, BK, WM, WN, TA, TB, EDGE>),
+            &n, reinterpret_cast<const void*>(&gemm_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC, TA, TB, EDGE>),
             Sh::NT, 0) != hipSuccess)
       return 0;
     return n;

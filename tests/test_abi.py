@@ -52,7 +52,11 @@ def test_tile_table_and_heuristic():
             assert c["BM"] % c["WM"] == 0 and c["BN"] % c["WN"] == 0
             assert c["threads"] == 64 * (c["BM"] // c["WM"]) * (c["BN"] // c["WN"])
             assert c["lds_bytes"] <= 160 * 1024
-    assert cucosma_amd.heuristic(0, 8192, 8192, 8192) == (0, 1)
+    for mode in range(4):
+        cfg, split = cucosma_amd.heuristic(mode, 8192, 8192, 8192)
+        t = cucosma_amd.tile_configs(mode)[cfg]
+        assert split == 1 and t["BM"] * t["BN"] >= 128 * 128
+        assert (8192 // t["BM"]) * (8192 // t["BN"]) >= 512
     cfg, split = cucosma_amd.heuristic(0, 256, 256, 1 << 20)
     assert 8 <= split <= 64
     assert cucosma_amd.heuristic(1, 32768, 1024, 1024)[1] == 1
