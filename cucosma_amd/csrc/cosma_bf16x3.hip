@@ -1,6 +1,12 @@
 // cosma_bf16x3.hip -- split-precision SGEMM (bf16 hi/lo, three products on
 // v_mfma_f32_16x16x32_bf16, FP32 accumulation): tile configurations.
 // Tile<mode, BM, BN, BK, WM, WN, PIPE, OCC> as in cosma_f32.hip.
+// cfg 0: 128x128, 4 waves of 64x64
+// cfg 1: 64x64, 4 waves of 32x32
+// cfg 2: 256x128, 8 waves of 64x64
+// cfg 3: 256x256, 8 waves of 128x64 (large shapes; 160 KiB of LDS)
+// cfg 4: 256x128, 4 waves of 128x64
+// cfg 5: 128x128 with a 64-deep K chunk (deep-K split)
 #include "cosma_launch.hpp"
 
 namespace cosma {

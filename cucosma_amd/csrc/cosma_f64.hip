@@ -1,5 +1,13 @@
 // cosma_f64.hip -- FP64 GEMM on v_mfma_f64_16x16x4_f64: tile configurations.
 // Tile<mode, BM, BN, BK, WM, WN, PIPE, OCC> as in cosma_f32.hip.
+// cfg 0: 128x128, 4 waves of 64x64
+// cfg 1: 64x64, 4 waves of 32x32
+// cfg 2: 128x64, 4 waves of 64x32
+// cfg 3: 128x128, 8 waves of 64x32, registers capped for 4 waves per SIMD
+//        (two workgroups per CU): the large-shape tile
+// cfg 4: 64x64 with a 32-deep K chunk
+// cfg 5: 128x128 with a 32-deep K chunk, 8 waves of 64x32
+// cfg 6: 256x128, 8 waves of 64x64
 #include "cosma_launch.hpp"
 
 namespace cosma {
