@@ -43,17 +43,20 @@ struct TileDims {
 constexpr int kMaxCfg = 8;
 constexpr TileDims kDims[4][kMaxCfg] = {
     // FP32
-    {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}, {256, 128, 32}},
+    {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}, {256, 128, 32},
+     {256, 128, 32}},
     // FP64
     {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 16}, {64, 64, 32}, {128, 128, 32},
-     {256, 128, 16}},
+     {256, 128, 16}, {128, 128, 16}},
     // BF16X3
-    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64}},
+    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
+     {256, 256, 32}},
     // BF16X4 (same tiles as BF16X3)
-    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64}},
+    {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
+     {256, 256, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {6, 7, 6, 6};
+constexpr int kNumCfg[kNumModes] = {7, 8, 7, 7};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -65,10 +68,11 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
 //  - large outputs (at least two workgroups per CU on the mode's large tile):
-//    FP32 256x128 (8 waves of 64x64, next-stage copy between the MFMAs), FP64
-//    128x128 (8 waves of 64x32), BF16x3 and BF16x4 256x256 (8 waves of
-//    128x64; BF16x4 with the copy between the MFMAs, which frees the
-//    registers its per-k-step temporaries need);
+//    FP32 256x128 (8 waves of 64x64, next-stage copy between the MFMAs);
+//    FP64 128x128 (8 waves of 64x32, 4 waves per SIMD, ping-pong copies);
+//    BF16x3 256x256 (8 waves of 128x64, ping-pong copies); BF16x4 256x256
+//    (copy between the MFMAs, which leaves room for its per-k-step
+//    temporaries);
 //  - deep-K outputs (few tiles, K >= 16384): split K over a 128-row tile with
 //    a long K chunk -- 128x64x32 (FP32/FP64) or 128x128x64 (BF16) -- until
 //    every CU holds two (FP32/FP64) or one (BF16, 144 KiB of LDS) workgroups;
@@ -80,7 +84,7 @@ struct Choice {
   int cfg, split;
 };
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
-constexpr int large_cfg(int mode) { return 3; }
+constexpr int large_cfg(int mode) { return mode == 0 ? 3 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 constexpr int deep_cfg(int mode) { return split_mode(mode) ? 5 : 2; }
 constexpr int clamp_split(int split, int k, int min_depth) {

@@ -4,9 +4,10 @@
 // cfg 0: 128x128, 4 waves of 64x64
 // cfg 1: 64x64, 4 waves of 32x32
 // cfg 2: 256x128, 8 waves of 64x64
-// cfg 3: 256x256, 8 waves of 128x64 (large shapes; 160 KiB of LDS)
+// cfg 3: 256x256, 8 waves of 128x64 (160 KiB of LDS)
 // cfg 4: 256x128, 4 waves of 128x64
 // cfg 5: 128x128 with a 64-deep K chunk (deep-K split)
+// cfg 6: cfg 3 with ping-pong copies (PIPE 4): the large-shape tile
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -17,9 +18,10 @@ using B3C2 = Tile<kBF16x3, 256, 128, 32, 64, 64>;
 using B3C3 = Tile<kBF16x3, 256, 256, 32, 128, 64>;
 using B3C4 = Tile<kBF16x3, 256, 128, 32, 128, 64>;
 using B3C5 = Tile<kBF16x3, 128, 128, 64, 64, 64>;
+using B3C6 = Tile<kBF16x3, 256, 256, 32, 128, 64, 4>;
 
-extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info()};
-extern const int kBF16x3NumTiles = 6;
+extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info(), B3C6::info()};
+extern const int kBF16x3NumTiles = 7;
 
 hipError_t run_gemm_bf16x3(const Problem& p) {
   switch (p.cfg) {
@@ -29,6 +31,7 @@ hipError_t run_gemm_bf16x3(const Problem& p) {
     case 3: return B3C3::run(p);
     case 4: return B3C4::run(p);
     case 5: return B3C5::run(p);
+    case 6: return B3C6::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -42,6 +45,7 @@ hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, 
     case 3: return B3C3::grouped(table, count, total, blocks, s);
     case 4: return B3C4::grouped(table, count, total, blocks, s);
     case 5: return B3C5::grouped(table, count, total, blocks, s);
+    case 6: return B3C6::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -54,6 +58,7 @@ int grouped_occupancy_bf16x3(int cfg) {
     case 3: return B3C3::grouped_occupancy();
     case 4: return B3C4::grouped_occupancy();
     case 5: return B3C5::grouped_occupancy();
+    case 6: return B3C6::grouped_occupancy();
     default: return 1;
   }
 }
@@ -66,6 +71,7 @@ int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge) {
     case 3: return B3C3::occupancy(ta, tb, edge);
     case 4: return B3C4::occupancy(ta, tb, edge);
     case 5: return B3C5::occupancy(ta, tb, edge);
+    case 6: return B3C6::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

@@ -2,7 +2,8 @@
 // hi*hi + hi*lo + lo*hi + lo*lo on v_mfma_f32_16x16x32_bf16, FP32
 // accumulation): the tiles of cosma_bf16x3.hip; cfg 3 places the next-stage
 // copy between the MFMAs (PIPE 1), which leaves room for the per-k-step
-// temporaries without spilling.
+// temporaries without spilling; cfg 6 (ping-pong) spills and is kept only
+// for completeness of the table.
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -13,9 +14,10 @@ using B4C2 = Tile<kBF16x4, 256, 128, 32, 64, 64>;
 using B4C3 = Tile<kBF16x4, 256, 256, 32, 128, 64, 1>;
 using B4C4 = Tile<kBF16x4, 256, 128, 32, 128, 64>;
 using B4C5 = Tile<kBF16x4, 128, 128, 64, 64, 64>;
+using B4C6 = Tile<kBF16x4, 256, 256, 32, 128, 64, 4>;
 
-extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info()};
-extern const int kBF16x4NumTiles = 6;
+extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info(), B4C6::info()};
+extern const int kBF16x4NumTiles = 7;
 
 hipError_t run_gemm_bf16x4(const Problem& p) {
   switch (p.cfg) {
@@ -25,6 +27,7 @@ hipError_t run_gemm_bf16x4(const Problem& p) {
     case 3: return B4C3::run(p);
     case 4: return B4C4::run(p);
     case 5: return B4C5::run(p);
+    case 6: return B4C6::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -38,6 +41,7 @@ hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, int total, 
     case 3: return B4C3::grouped(table, count, total, blocks, s);
     case 4: return B4C4::grouped(table, count, total, blocks, s);
     case 5: return B4C5::grouped(table, count, total, blocks, s);
+    case 6: return B4C6::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -50,6 +54,7 @@ int grouped_occupancy_bf16x4(int cfg) {
     case 3: return B4C3::grouped_occupancy();
     case 4: return B4C4::grouped_occupancy();
     case 5: return B4C5::grouped_occupancy();
+    case 6: return B4C6::grouped_occupancy();
     default: return 1;
   }
 }
@@ -62,6 +67,7 @@ int gemm_occupancy_bf16x4(int cfg, int ta, int tb, bool edge) {
     case 3: return B4C3::occupancy(ta, tb, edge);
     case 4: return B4C4::occupancy(ta, tb, edge);
     case 5: return B4C5::occupancy(ta, tb, edge);
+    case 6: return B4C6::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

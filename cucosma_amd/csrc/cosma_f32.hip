@@ -8,6 +8,7 @@
 // cfg 3: 256x128, 8 waves of 64x64 (large shapes: 134 TFLOP/s at 8192^3)
 // cfg 4: 128x128 with a 64-deep K chunk
 // cfg 5: cfg 3 with register double-buffered prefetch (PIPE 3)
+// cfg 6: cfg 3 with ping-pong copies (PIPE 4)
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -18,9 +19,10 @@ using F32C2 = Tile<kF32, 128, 64, 32, 64, 32, 1>;
 using F32C3 = Tile<kF32, 256, 128, 32, 64, 64, 1>;
 using F32C4 = Tile<kF32, 128, 128, 64, 64, 64, 1>;
 using F32C5 = Tile<kF32, 256, 128, 32, 64, 64, 3>;
+using F32C6 = Tile<kF32, 256, 128, 32, 64, 64, 4>;
 
-extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info(), F32C4::info(), F32C5::info()};
-extern const int kF32NumTiles = 6;
+extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info(), F32C4::info(), F32C5::info(), F32C6::info()};
+extern const int kF32NumTiles = 7;
 
 hipError_t run_gemm_f32(const Problem& p) {
   switch (p.cfg) {
@@ -30,6 +32,7 @@ hipError_t run_gemm_f32(const Problem& p) {
     case 3: return F32C3::run(p);
     case 4: return F32C4::run(p);
     case 5: return F32C5::run(p);
+    case 6: return F32C6::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -43,6 +46,7 @@ hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int
     case 3: return F32C3::grouped(table, count, total, blocks, s);
     case 4: return F32C4::grouped(table, count, total, blocks, s);
     case 5: return F32C5::grouped(table, count, total, blocks, s);
+    case 6: return F32C6::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -55,6 +59,7 @@ int grouped_occupancy_f32(int cfg) {
     case 3: return F32C3::grouped_occupancy();
     case 4: return F32C4::grouped_occupancy();
     case 5: return F32C5::grouped_occupancy();
+    case 6: return F32C6::grouped_occupancy();
     default: return 1;
   }
 }
@@ -67,6 +72,7 @@ int gemm_occupancy_f32(int cfg, int ta, int tb, bool edge) {
     case 3: return F32C3::occupancy(ta, tb, edge);
     case 4: return F32C4::occupancy(ta, tb, edge);
     case 5: return F32C5::occupancy(ta, tb, edge);
+    case 6: return F32C6::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

@@ -4,10 +4,11 @@
 // cfg 1: 64x64, 4 waves of 32x32
 // cfg 2: 128x64, 4 waves of 64x32
 // cfg 3: 128x128, 8 waves of 64x32, registers capped for 4 waves per SIMD
-//        (two workgroups per CU): the large-shape tile
+//        (two workgroups per CU)
 // cfg 4: 64x64 with a 32-deep K chunk
 // cfg 5: 128x128 with a 32-deep K chunk, 8 waves of 64x32
 // cfg 6: 256x128, 8 waves of 64x64
+// cfg 7: cfg 3 with ping-pong copies (PIPE 4): the large-shape tile
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -19,9 +20,10 @@ using F64C3 = Tile<kF64, 128, 128, 16, 64, 32, 0, 4>;
 using F64C4 = Tile<kF64, 64, 64, 32, 32, 32>;
 using F64C5 = Tile<kF64, 128, 128, 32, 64, 32>;
 using F64C6 = Tile<kF64, 256, 128, 16, 64, 64>;
+using F64C7 = Tile<kF64, 128, 128, 16, 64, 32, 4, 4>;
 
-extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info(), F64C3::info(), F64C4::info(), F64C5::info(), F64C6::info()};
-extern const int kF64NumTiles = 7;
+extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info(), F64C3::info(), F64C4::info(), F64C5::info(), F64C6::info(), F64C7::info()};
+extern const int kF64NumTiles = 8;
 
 hipError_t run_gemm_f64(const Problem& p) {
   switch (p.cfg) {
@@ -32,6 +34,7 @@ hipError_t run_gemm_f64(const Problem& p) {
     case 4: return F64C4::run(p);
     case 5: return F64C5::run(p);
     case 6: return F64C6::run(p);
+    case 7: return F64C7::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -46,6 +49,7 @@ hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int
     case 4: return F64C4::grouped(table, count, total, blocks, s);
     case 5: return F64C5::grouped(table, count, total, blocks, s);
     case 6: return F64C6::grouped(table, count, total, blocks, s);
+    case 7: return F64C7::grouped(table, count, total, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -59,6 +63,7 @@ int grouped_occupancy_f64(int cfg) {
     case 4: return F64C4::grouped_occupancy();
     case 5: return F64C5::grouped_occupancy();
     case 6: return F64C6::grouped_occupancy();
+    case 7: return F64C7::grouped_occupancy();
     default: return 1;
   }
 }
@@ -72,6 +77,7 @@ int gemm_occupancy_f64(int cfg, int ta, int tb, bool edge) {
     case 4: return F64C4::occupancy(ta, tb, edge);
     case 5: return F64C5::occupancy(ta, tb, edge);
     case 6: return F64C6::occupancy(ta, tb, edge);
+    case 7: return F64C7::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

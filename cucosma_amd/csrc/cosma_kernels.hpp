@@ -337,7 +337,9 @@ struct TileShape {
 //      global loads to land);
 //   3  as 1, with mid() after the first quarter of the rows; the caller
 //      double-buffers the loader registers so mid()'s data arrived an
-//      iteration earlier (gemm_tile).
+//      iteration earlier (gemm_tile);
+//   4  ping-pong (gemm_tile): half the waves copy before their MFMAs, half
+//      after, each calling this with the PIPE 0 order.
 // Which is faster depends on the tile (register headroom, MFMA latency of the
 // type); each tile configuration fixes its choice from measurement.
 template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, typename Mid>
@@ -576,6 +578,45 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
       step(it + 1, la1, lb1, la, lb);
     }
     if (it < nk) step(it, la, lb, la1, lb1);
+  } else if constexpr (PIPE == 4) {
+    // Ping-pong: waves in the upper half of the workgroup (one per SIMD for
+    // 8 waves) copy first -- they write chunk it+1, loaded during the previous
+    // iteration, then load chunk it+2 into the freed registers, then run the
+    // MFMAs of chunk it; the lower half loads chunk it+1, runs its MFMAs and
+    // copies last.  The two kinds of wave on a SIMD do their copies at
+    // opposite ends of the iteration, so one wave's VALU/LDS work overlaps
+    // the other's MFMAs.  Both write stage it+1 before the barrier and read
+    // stage it after the previous one.
+    const bool copy_first = wave >= Sh::WAVES_M * Sh::WAVES_N / 2;
+    if (nk > 0) {
+      la.load(0, p.M, kend, tid, p.vec_a);
+      lb.load(0, p.N, kend, tid, p.vec_b);
+      store_to(la, lb, stage_ptr(0));
+      if (copy_first) {
+        const int k1 = (nk > 1 ? 1 : 0) * BK;
+        la.load(k1, p.M, kend, tid, p.vec_a);
+        lb.load(k1, p.N, kend, tid, p.vec_b);
+      }
+      __syncthreads();
+    }
+    for (int it = 0; it < nk; ++it) {
+      const int cur = it & 1;
+      S* nxt = stage_ptr(cur ^ 1);
+      if (copy_first) {
+        store_to(la, lb, nxt);
+        const int kofs = min(it + 2, nk - 1) * BK;
+        la.load(kofs, p.M, kend, tid, p.vec_a);
+        lb.load(kofs, p.N, kend, tid, p.vec_b);
+        compute_stage<MODE, BM, BN, BK, WM, WN, 0>(stage_ptr(cur), acc, wm0, wn0, lane, []() {});
+      } else {
+        const int kofs = min(it + 1, nk - 1) * BK;
+        la.load(kofs, p.M, kend, tid, p.vec_a);
+        lb.load(kofs, p.N, kend, tid, p.vec_b);
+        compute_stage<MODE, BM, BN, BK, WM, WN, 0>(stage_ptr(cur), acc, wm0, wn0, lane,
+                                                   [&]() { store_to(la, lb, nxt); });
+      }
+      __syncthreads();
+    }
   } else {
     if (nk > 0) {
       la.load(0, p.M, kend, tid, p.vec_a);
