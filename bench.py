@@ -163,7 +163,7 @@ def main() -> int:
     cfg_b, split_b = cc.heuristic(cc.MODE_BF16X3, M, N, K)
     tile_b = cc.tile_configs(cc.MODE_BF16X3)[cfg_b]
     kname = (f"gemm_kernel<BF16x3, {tile_b['BM']}x{tile_b['BN']}x{tile_b['BK']}, "
-             f"{tile_b['threads'] // 64} waves of {tile_b['WM']}x{tile_b['WN']}>")
+             f"{tile_b['threads'] // 64} waves of {tile_b['WM']}x{tile_b['WN']}, pipe {tile_b['pipe']}>")
     tf_b = flops / wall_b / 1e9
     tf_p = flops / wall_p / 1e9
     tf_4 = flops / wall_4 / 1e9

@@ -156,11 +156,12 @@ def check(status: int, what: str = "") -> None:
 def tile_configs(mode: int) -> list[dict]:
     """Tile configurations built into the library for a mode."""
     out = []
-    arr = (C.c_int * 7)()
+    arr = (C.c_int * 8)()
     for cfg in range(lib.cosma_num_tile_configs(mode)):
         check(lib.cosma_tile_config(mode, cfg, arr), "cosma_tile_config")
-        bm, bn, bk, wm, wn, thr, lds = list(arr)
-        out.append(dict(cfg=cfg, BM=bm, BN=bn, BK=bk, WM=wm, WN=wn, threads=thr, lds_bytes=lds))
+        bm, bn, bk, wm, wn, thr, lds, pipe = list(arr)
+        out.append(dict(cfg=cfg, BM=bm, BN=bn, BK=bk, WM=wm, WN=wn, threads=thr, lds_bytes=lds,
+                        pipe=pipe))
     return out
 
 

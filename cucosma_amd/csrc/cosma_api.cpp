@@ -640,20 +640,21 @@ int cosma_num_tile_configs(int mode) {
   return kNumCfg[mode];
 }
 
-int cosma_tile_config(int mode, int cfg, int* out7) {
-  if (!out7 || mode < 0 || mode >= kNumModes || cfg < 0 || cfg >= kNumCfg[mode])
+int cosma_tile_config(int mode, int cfg, int* out8) {
+  if (!out8 || mode < 0 || mode >= kNumModes || cfg < 0 || cfg >= kNumCfg[mode])
     return COSMA_STATUS_INVALID_VALUE;
   const cosma::TileInfo* t = cosma::tile_info(mode, cfg);
   if (!t || t->BM != kDims[mode][cfg].bm || t->BN != kDims[mode][cfg].bn ||
       t->BK != kDims[mode][cfg].bk)
     return COSMA_STATUS_INVALID_VALUE;
-  out7[0] = t->BM;
-  out7[1] = t->BN;
-  out7[2] = t->BK;
-  out7[3] = t->WM;
-  out7[4] = t->WN;
-  out7[5] = t->threads;
-  out7[6] = t->lds_bytes;
+  out8[0] = t->BM;
+  out8[1] = t->BN;
+  out8[2] = t->BK;
+  out8[3] = t->WM;
+  out8[4] = t->WN;
+  out8[5] = t->threads;
+  out8[6] = t->lds_bytes;
+  out8[7] = t->pipe;
   return 0;
 }
 

@@ -35,7 +35,10 @@ struct TileInfo {
   int BM, BN, BK, WM, WN;
   int threads;
   int lds_bytes;
-  int pipe;  // 0: next-stage copy after the MFMAs, 1: between them
+  // Next-stage copy placement: 0 after the MFMAs, 1 after half of them, 2
+  // after three quarters, 3 register double-buffered, 4 ping-pong (half the
+  // waves copy before their MFMAs, half after).
+  int pipe;
 };
 
 // One problem of a grouped launch, as the grouped kernel reads it from

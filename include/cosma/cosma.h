@@ -135,11 +135,14 @@ int cosma_dgemm_mnk(int m, int n, int k, double alpha, const double *A, int lda,
 int cosma_sgemm_bf16x3_mnk(int m, int n, int k, float alpha, const float *A, int lda,
                            const float *B, int ldb, float beta, float *C, int ldc, void *stream);
 
-/* Tile configurations built into the library.  out[0..6] = BM, BN, BK, WM,
- * WN, threads per workgroup, LDS bytes.  Replaces the tile macros of
- * config.h (THREADBLOCK_TILE_*, WARP_TILE_*, LOAD_K). */
+/* Tile configurations built into the library.  out[0..7] = BM, BN, BK, WM,
+ * WN, threads per workgroup, LDS bytes, pipeline schedule (0: next-stage copy
+ * after the MFMAs, 1: after half of them, 2: after three quarters, 3:
+ * register double-buffered, 4: ping-pong -- half the waves copy before their
+ * MFMAs, half after).  Replaces the tile
+ * macros of config.h (THREADBLOCK_TILE_*, WARP_TILE_*, LOAD_K). */
 int cosma_num_tile_configs(int mode);
-int cosma_tile_config(int mode, int cfg, int *out7);
+int cosma_tile_config(int mode, int cfg, int *out8);
 /* The (tile_cfg, split_k) the library heuristic picks for a shape. */
 int cosma_heuristic(int mode, int m, int n, int k, int *tile_cfg, int *split_k);
 
