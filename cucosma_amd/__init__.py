@@ -10,6 +10,7 @@ from ._lib import (  # noqa: F401
     MODE_BF16X4,
     MODE_FP32,
     MODE_FP64,
+    EPILOGUES,
     CosmaError,
     check,
     heuristic,

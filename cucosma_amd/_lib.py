@@ -23,6 +23,12 @@ MODE_BF16X3 = 2
 MODE_BF16X4 = 3
 MODES = {"fp32": MODE_FP32, "fp64": MODE_FP64, "bf16x3": MODE_BF16X3, "bf16x4": MODE_BF16X4}
 
+EPILOGUE_NONE = 0
+EPILOGUE_RELU = 1
+EPILOGUE_SIGMOID = 2
+EPILOGUES = {None: EPILOGUE_NONE, "none": EPILOGUE_NONE, "relu": EPILOGUE_RELU,
+             "sigmoid": EPILOGUE_SIGMOID}
+
 
 class CosmaError(RuntimeError):
     """A non-zero status returned by the C ABI."""
@@ -57,6 +63,7 @@ class GemmDesc(C.Structure):
         ("workspace", C.c_void_p),
         ("workspace_bytes", C.c_size_t),
         ("stream", C.c_void_p),
+        ("epilogue", C.c_int),
     ]
 
 

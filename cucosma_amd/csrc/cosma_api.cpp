@@ -73,9 +73,13 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    BF16x3 256x256 (8 waves of 128x64, ping-pong copies); BF16x4 256x256
 //    (copy between the MFMAs, which leaves room for its per-k-step
 //    temporaries);
-//  - deep-K outputs (few tiles, K >= 16384): split K over a 128-row tile with
-//    a long K chunk -- 128x64x32 (FP32/FP64) or 128x128x64 (BF16) -- until
-//    every CU holds two (FP32/FP64) or one (BF16, 144 KiB of LDS) workgroups;
+//  - deep-K outputs (few tiles, K >= 16384): split K over the large tile --
+//    FP32 256x128 and BF16x3 256x256 with ping-pong copies, BF16x4 256x256 --
+//    until every CU holds one workgroup (each K slice at least 512 deep; at
+//    256x256x1048576 that is 256 slices of 4096, so every input byte is read
+//    from HBM once); FP64, and outputs smaller than the large tile, split a
+//    128-row tile with a long K chunk (128x64x32, or 128x128x64 for BF16)
+//    until every CU holds two (128x64) or one (128x128x64) workgroups;
 //  - otherwise 128x128 (BF16) / 128x64 tiles while they give every CU a
 //    workgroup, else 64x64; split K when the 64x64 grid still leaves CUs idle
 //    (FP32/FP64) or gives them fewer than two workgroups (BF16), keeping every
@@ -86,21 +90,27 @@ struct Choice {
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
 constexpr int large_cfg(int mode) { return mode == 0 ? 3 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
-constexpr int deep_cfg(int mode) { return split_mode(mode) ? 5 : 2; }
-constexpr int clamp_split(int split, int k, int min_depth) {
+// Deep-K tile: the large tile when the output fills it, else a 128-row tile
+// with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
+constexpr int deep_cfg(int mode, int m, int n) {
+  const int big = mode == 0 ? 6 : (mode == 1 ? 2 : (mode == 2 ? 6 : 3));
+  if (m >= kDims[mode][big].bm && n >= kDims[mode][big].bn) return big;
+  return split_mode(mode) ? 5 : 2;
+}
+constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   int max_by_depth = k / min_depth;
   if (split > max_by_depth) split = max_by_depth;
-  if (split > 64) split = 64;
+  if (split > max_split) split = max_split;
   return split < 1 ? 1 : split;
 }
 constexpr Choice heuristic(int mode, int m, int n, int k) {
   int cfg = large_cfg(mode);
   if (tiles_of(mode, cfg, m, n) >= 2 * kCUs) return Choice{cfg, 1};
-  const int dc = deep_cfg(mode);
+  const int dc = deep_cfg(mode, m, n);
   const int dtiles = tiles_of(mode, dc, m, n);
   if (k >= 16384 && dtiles < kCUs) {
-    const int per_cu = split_mode(mode) ? 1 : 2;
-    return Choice{dc, clamp_split(cdiv(per_cu * kCUs, dtiles), k, 2 * kDims[mode][dc].bk)};
+    const int per_cu = (mode == 1 || dc == 2) ? 2 : 1;
+    return Choice{dc, clamp_split(cdiv(per_cu * kCUs, dtiles), k, 512, kCUs)};
   }
   cfg = medium_cfg(mode);
   if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;  // 64x64
@@ -203,6 +213,8 @@ int validate(const cosma_gemm_desc* d) {
     return COSMA_STATUS_INVALID_VALUE;
   if (d->split_k > kMaxSplit) return COSMA_STATUS_INVALID_VALUE;
   if (d->tile_cfg >= kNumCfg[d->mode]) return COSMA_STATUS_INVALID_VALUE;
+  if (d->epilogue < COSMA_EPILOGUE_NONE || d->epilogue > COSMA_EPILOGUE_SIGMOID)
+    return COSMA_STATUS_INVALID_VALUE;
   if (d->m == 0 || d->n == 0) return 0;
   if (!d->c) return COSMA_STATUS_INVALID_VALUE;
   if (d->ldc < (d->n > 1 ? d->n : 1)) return COSMA_STATUS_INVALID_VALUE;
@@ -225,12 +237,13 @@ int run(const cosma_gemm_desc* d) {
   if (d->m == 0 || d->n == 0) return 0;
   hipStream_t s = static_cast<hipStream_t>(d->stream);
   const bool uses_ab = d->k > 0 && d->alpha != 0.0;
-  if (!uses_ab) {  // C = beta*C (reference: pre-scale only, cuCOSMAV100.cuh:2782-2797)
-    if (d->beta == 1.0) return 0;
+  if (!uses_ab) {  // C = act(beta*C) (reference: pre-scale only, cuCOSMAV100.cuh:2782-2797)
+    if (d->beta == 1.0 && d->epilogue == COSMA_EPILOGUE_NONE) return 0;
     hipError_t e = d->mode == COSMA_MODE_FP64
-                       ? cosma::run_scale_f64(static_cast<double*>(d->c), d->ldc, d->m, d->n, d->beta, s)
+                       ? cosma::run_scale_f64(static_cast<double*>(d->c), d->ldc, d->m, d->n, d->beta,
+                                              d->epilogue, s)
                        : cosma::run_scale_f32(static_cast<float*>(d->c), d->ldc, d->m, d->n,
-                                              (float)d->beta, s);
+                                              (float)d->beta, d->epilogue, s);
     return (int)e;
   }
   Choice ch = heuristic(d->mode, d->m, d->n, d->k);
@@ -261,6 +274,7 @@ int run(const cosma_gemm_desc* d) {
   p.split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
   if (p.split > d->k) p.split = d->k > 0 ? d->k : 1;
   p.group_m = group_m;
+  p.act = d->epilogue;
   p.ws = nullptr;
   p.stream = s;
   if (p.split > 1) {
@@ -547,6 +561,7 @@ void cosma_gemm_desc_init(cosma_gemm_desc* d) {
   d->workspace = nullptr;
   d->workspace_bytes = 0;
   d->stream = nullptr;
+  d->epilogue = COSMA_EPILOGUE_NONE;
 }
 
 int cosma_gemm_ex(const cosma_gemm_desc* d) { return run(d); }
@@ -803,12 +818,12 @@ int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_re
   for (int cfg = cfg_lo; cfg < cfg_hi; ++cfg) {
     const TileDims td = kDims[d->mode][cfg];
     const int tiles = tiles_of(d->mode, cfg, d->m, d->n);
-    for (int split = 1; split <= 64; split *= 2) {
+    for (int split = 1; split <= 1024; split *= 2) {
       if (d->split_k > 0 && split != d->split_k) continue;
       if (split > 1 && (split * 2 * td.bk > d->k || (int64_t)tiles * split > 16 * kCUs)) continue;
       try_one(cfg, split, gm0);
     }
-    if (d->split_k > 64) try_one(cfg, d->split_k, gm0);
+    if (d->split_k > 1024) try_one(cfg, d->split_k, gm0);
   }
   auto best_of = [&]() {
     int b = -1;

@@ -5,14 +5,15 @@
 
 namespace cosma {
 
-// Fixed-order split-K reduction: C = alpha * sum_{z=0..S-1} ws[z] (+ beta*C).
+// Fixed-order split-K reduction: C = act(alpha * sum_{z=0..S-1} ws[z] (+ beta*C)).
 // Every element sums its partials in ascending z, so the result does not
 // depend on launch order or timing (Kernel/cuda/src/cosmaSplitKReduce.cuh:20-42
-// sums in the same order).  beta == 0 does not read C.
+// sums in the same order).  beta == 0 does not read C.  The activation runs
+// here, on the reduced value, not on each slice's partial.
 template <typename E>
 __global__ void __launch_bounds__(256)
     splitk_reduce_kernel(const E* __restrict__ ws, E* __restrict__ C, int64_t ldc, int M, int N,
-                         int S, E alpha, E beta) {
+                         int S, E alpha, E beta, int act) {
   const int64_t total = (int64_t)M * N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
@@ -23,22 +24,24 @@ __global__ void __launch_bounds__(256)
     E* cp = C + (int64_t)row * ldc + col;
     E out = alpha * v;
     if (beta != E(0)) out = fma_e(beta, *cp, out);
+    if (act != kActNone) out = activate(out, act);
     *cp = out;
   }
 }
 
-// C = beta * C over the M x N window (alpha == 0 or K == 0); beta == 0 writes
-// zeros without reading C.
+// C = act(beta * C) over the M x N window (alpha == 0 or K == 0); beta == 0
+// writes act(0) without reading C.
 template <typename E>
 __global__ void __launch_bounds__(256)
-    scale_kernel(E* __restrict__ C, int64_t ldc, int M, int N, E beta) {
+    scale_kernel(E* __restrict__ C, int64_t ldc, int M, int N, E beta, int act) {
   const int64_t total = (int64_t)M * N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
     const int row = (int)(e / N);
     const int col = (int)(e - (int64_t)row * N);
     E* cp = C + (int64_t)row * ldc + col;
-    *cp = beta == E(0) ? E(0) : beta * *cp;
+    const E v = beta == E(0) ? E(0) : beta * *cp;
+    *cp = act != kActNone ? activate(v, act) : v;
   }
 }
 
@@ -216,27 +219,28 @@ static int grid_for(int64_t total, int threads, int cap) {
   return (int)(b < cap ? b : cap);
 }
 
-hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, hipStream_t s) {
+hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, int act, hipStream_t s) {
   hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for((int64_t)M * N, 256, 8192)), dim3(256), 0,
-                     s, C, ldc, M, N, beta);
+                     s, C, ldc, M, N, beta, act);
   return hipGetLastError();
 }
-hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, hipStream_t s) {
+hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, int act,
+                         hipStream_t s) {
   hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for((int64_t)M * N, 256, 8192)), dim3(256), 0,
-                     s, C, ldc, M, N, beta);
+                     s, C, ldc, M, N, beta, act);
   return hipGetLastError();
 }
 
 hipError_t run_reduce_f32(const float* ws, float* C, int64_t ldc, int M, int N, int S, float alpha,
-                          float beta, hipStream_t s) {
+                          float beta, int act, hipStream_t s) {
   hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(grid_for((int64_t)M * N, 256, 8192)),
-                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta);
+                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta, act);
   return hipGetLastError();
 }
 hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N, int S,
-                          double alpha, double beta, hipStream_t s) {
+                          double alpha, double beta, int act, hipStream_t s) {
   hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(grid_for((int64_t)M * N, 256, 8192)),
-                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta);
+                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta, act);
   return hipGetLastError();
 }
 

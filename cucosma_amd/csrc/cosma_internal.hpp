@@ -26,6 +26,7 @@ struct Problem {
   int split;
   int cfg;
   int group_m;
+  int act;  // output activation: 0 none, 1 ReLU, 2 sigmoid
   void* ws;
   hipStream_t stream;
 };
@@ -76,15 +77,16 @@ int grouped_occupancy(int mode, int cfg);
 // query fails or the layout is not built).
 int gemm_occupancy(int mode, int cfg, int ta, int tb, bool edge);
 
-// C = beta*C over the M x N window (alpha == 0 or K == 0).
-hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, hipStream_t s);
-hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, hipStream_t s);
+// C = act(beta*C) over the M x N window (alpha == 0 or K == 0).
+hipError_t run_scale_f32(float* C, int64_t ldc, int M, int N, float beta, int act, hipStream_t s);
+hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, int act,
+                         hipStream_t s);
 
-// Fixed-order split-K reduction.
+// Fixed-order split-K reduction, then the activation.
 hipError_t run_reduce_f32(const float* ws, float* C, int64_t ldc, int M, int N, int S, float alpha,
-                          float beta, hipStream_t s);
+                          float beta, int act, hipStream_t s);
 hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N, int S,
-                          double alpha, double beta, hipStream_t s);
+                          double alpha, double beta, int act, hipStream_t s);
 
 // Overwrite `bytes` of buf (cache flush between timed runs).
 hipError_t run_flush(void* buf, size_t bytes, hipStream_t s);

@@ -120,8 +120,24 @@ struct GemmParams {
   int group_m; // tile rows per group in the grouped tile order
   int vec_a;   // 1: A's rows start on 16-byte boundaries (vector loads allowed)
   int vec_b;   // 1: same for B
+  int act;     // element-wise activation applied to the final C values (kActNone/ReLU/Sigmoid)
   E alpha, beta;
 };
+
+// Element-wise activations of the output (fused epilogue).  The reference has
+// them as commented-out loops over the thread tile in store_C
+// (Kernel/cuda/src/cuCOSMAV100.cuh:2466-2476: fmaxf(x, 0) and
+// 1 / (1 + expf(-x))) and as a separate pass (Kernel/cuda/src/sigmoid4.cuh:13-27).
+// Here they act on alpha*AB + beta*C, after the split-K reduction.
+constexpr int kActNone = 0;
+constexpr int kActReLU = 1;
+constexpr int kActSigmoid = 2;
+template <typename E>
+__device__ __forceinline__ E activate(E v, int act) {
+  if (act == kActReLU) return fmax(v, E(0));  // NaN -> 0, as fmaxf
+  if (act == kActSigmoid) return E(1) / (E(1) + exp(-v));
+  return v;
+}
 
 // Split four floats into RNE bf16 hi parts and RNE bf16 lo = x - hi parts
 // (x - hi is exact in FP32).  v_cvt_pk_bf16_f32 does the rounding.
@@ -483,6 +499,7 @@ __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int r
   E* base = partial ? p.ws + ((int64_t)z * p.M + lrow) * p.N + lcol : p.C + (int64_t)lrow * p.ldc + lcol;
   const E scale = partial ? E(1) : p.alpha;
   const bool read_c = !partial && p.beta != E(0);
+  const int act = partial ? kActNone : p.act;
   const E beta = p.beta;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -495,6 +512,7 @@ __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int r
         if (EDGE && (lrow + ro >= p.M || lcol + j * 16 >= p.N)) continue;
         E out = scale * acc[i][j][r];
         if (read_c) out = fma_e(beta, q[j * 16], out);
+        if (act != kActNone) out = activate(out, act);
         q[j * 16] = out;
       }
     }
@@ -684,6 +702,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
     p.B = e.B;
     p.C = e.C;
     p.ws = nullptr;
+    p.act = kActNone;
     p.lda = e.lda;
     p.ldb = e.ldb;
     p.ldc = e.ldc;

@@ -74,6 +74,7 @@ struct Tile {
     const int per = (pr.K + pr.split - 1) / pr.split;
     gp.kslice = ((per + BK - 1) / BK) * BK;
     gp.group_m = pr.group_m > 0 ? pr.group_m : 8;
+    gp.act = pr.act;
     gp.alpha = (E)pr.alpha;
     gp.beta = (E)pr.beta;
     const int64_t nblk64 = (int64_t)gp.tiles_m * gp.tiles_n * pr.split;
@@ -96,10 +97,10 @@ struct Tile {
     if (e != hipSuccess || pr.split == 1) return e;
     if constexpr (MODE == kF64)
       return run_reduce_f64(static_cast<const double*>(pr.ws), static_cast<double*>(pr.C), pr.ldc,
-                            pr.M, pr.N, pr.split, pr.alpha, pr.beta, pr.stream);
+                            pr.M, pr.N, pr.split, pr.alpha, pr.beta, pr.act, pr.stream);
     else
       return run_reduce_f32(static_cast<const float*>(pr.ws), static_cast<float*>(pr.C), pr.ldc,
-                            pr.M, pr.N, pr.split, (float)pr.alpha, (float)pr.beta, pr.stream);
+                            pr.M, pr.N, pr.split, (float)pr.alpha, (float)pr.beta, pr.act, pr.stream);
   }
 };
 

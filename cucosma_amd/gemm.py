@@ -11,7 +11,7 @@ import ctypes as C
 
 import torch
 
-from ._lib import MODES, GemmDesc, check, lib
+from ._lib import EPILOGUES, MODES, GemmDesc, check, lib
 
 _DTYPE = {"fp32": torch.float32, "bf16x3": torch.float32, "bf16x4": torch.float32,
           "fp64": torch.float64}
@@ -46,8 +46,9 @@ def gemm(
     group_m: int = 0,
     workspace: torch.Tensor | None = None,
     stream=None,
+    epilogue: str | None = None,
 ) -> torch.Tensor:
-    """C = alpha * op(A) @ op(B) + beta * C on the GPU.
+    """C = act(alpha * op(A) @ op(B) + beta * C) on the GPU.
 
     op(A) is A (shape M x K) or, with trans_a, A.T of a stored K x M tensor;
     op(B) is B (K x N) or, with trans_b, B.T of a stored N x K tensor.
@@ -55,9 +56,13 @@ def gemm(
     bf16 hi/lo split, three bf16 MFMA products, FP32 accumulation) or
     "bf16x4" (the same split with the lo*lo product added).
     split_k <= 0 and tile_cfg < 0 select the library heuristic.
+    epilogue: None, "relu" (max(x, 0)) or "sigmoid" (1 / (1 + exp(-x))), fused
+    into the kernel's store (after the split-K reduction when K is split).
     """
     if mode not in MODES:
         raise ValueError(f"unknown mode {mode!r}")
+    if epilogue not in EPILOGUES:
+        raise ValueError(f"unknown epilogue {epilogue!r}")
     dt = _DTYPE[mode]
     for t, name in ((A, "A"), (B, "B")):
         if t.dtype != dt:
@@ -84,6 +89,7 @@ def gemm(
     d.c, d.ldc = C_out.data_ptr(), _ld(C_out, "C")
     d.alpha, d.beta = float(alpha), float(beta)
     d.split_k, d.tile_cfg, d.group_m = int(split_k), int(tile_cfg), int(group_m)
+    d.epilogue = EPILOGUES[epilogue]
     if workspace is not None:
         d.workspace = workspace.data_ptr()
         d.workspace_bytes = workspace.numel() * workspace.element_size()

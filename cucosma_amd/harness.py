@@ -12,7 +12,11 @@ the HBM-bound flat shape, and normwise error against an FP64 product.
 Configurations:
   plain    1024^3, alpha 1, beta 0.5, A/B/C uniform[-1,1] from one MT19937(0) stream
   square   8192^3, A/B uniform[-1,1] seed 1, beta 0: fp32, bf16x3, bf16x4
-  flat     256 x 256 x 1,048,576, seed 1, beta 0, split-K 8..64: fp32, bf16x3, bf16x4
+  flat     256 x 256 x 1,048,576, seed 1, beta 0: fp32, bf16x3, bf16x4, the
+           library's tile with split-K 8..512 (its default split marked)
+  epilogue 8192^3 fp32 and bf16x3: plain, sigmoid fused into the store, and
+           plain followed by a separate sigmoid pass (the reference's two
+           variants, cuCOSMAV100.cuh:2466-2476 and sigmoid4.cuh:13-27)
   dgemm    32768x1024x1024 and 1024x65536x1024, normal(0,1) seed 2 with 1% of
            entries x1e6, C normal(0,1), alpha = beta = 1
   grouped  512 problems, M/N/K log-uniform in [32, 2048] rounded to 32, seed 3:
@@ -37,7 +41,7 @@ HBM_PEAK = 8000.0  # GB/s
 FLUSH_BYTES = 512 << 20
 
 
-def _desc(cc, mode, A, B, Cm, alpha, beta, split_k=0, tile_cfg=-1, group_m=0):
+def _desc(cc, mode, A, B, Cm, alpha, beta, split_k=0, tile_cfg=-1, group_m=0, epilogue=None):
     from ._lib import MODES, GemmDesc
     from .gemm import _ld, _stream_ptr
 
@@ -51,6 +55,7 @@ def _desc(cc, mode, A, B, Cm, alpha, beta, split_k=0, tile_cfg=-1, group_m=0):
     d.c, d.ldc = Cm.data_ptr(), _ld(Cm, "C")
     d.alpha, d.beta = alpha, beta
     d.split_k, d.tile_cfg, d.group_m = split_k, tile_cfg, group_m
+    d.epilogue = cc.EPILOGUES[epilogue]
     d.stream = _stream_ptr(None)
     return d
 
@@ -130,17 +135,55 @@ def cfg_flat(cc, torch, args, emit):
     R = cc.fp64_product(A, B)
     alg_bytes = (M * K + K * N + M * N) * 4
     for mode in ("fp32", "bf16x3", "bf16x4"):
-        for split in (8, 16, 32, 64, 128):
+        cfg, default_split = cc.heuristic(cc._lib.MODES[mode], M, N, K)
+        t = cc.tile_configs(cc._lib.MODES[mode])[cfg]
+        tiles = (M // t["BM"]) * (N // t["BN"])
+        for split in sorted({8, 16, 32, 64, 128, 256, 512, default_split}):
+            if tiles * split > 4096:
+                continue
             Cm = torch.empty((M, N), device="cuda")
-            cc.gemm(A, B, Cm, mode=mode, split_k=split, tile_cfg=1)
+            cc.gemm(A, B, Cm, mode=mode, split_k=split, tile_cfg=cfg)
             err = _err(cc, A, B, Cm, R)
             ms = bench_gemm(cc, mode, A, B, Cm, 1.0, 0.0, args.runs, args.warmup, not args.no_flush,
-                            split_k=split, tile_cfg=1)
+                            split_k=split, tile_cfg=cfg)
             med = float(np.median(ms))
             gbs = alg_bytes / (med * 1e-3) / 1e9
             _write_csv(args.csv_dir, M, N, K, f"cosma_{mode}_sk{split}", ms)
-            emit(_summary("flat", mode, M, N, K, ms, dict(err, split_k=split, tile="64x64",
-                                                           hbm_gbs=gbs, pct_hbm=100 * gbs / HBM_PEAK)))
+            emit(_summary("flat", mode, M, N, K, ms, dict(
+                err, split_k=split, tile_cfg=cfg, tile=f"{t['BM']}x{t['BN']}x{t['BK']}",
+                library_default=split == default_split, hbm_gbs=gbs, pct_hbm=100 * gbs / HBM_PEAK)))
+
+
+def cfg_epilogue(cc, torch, args, emit):
+    M = N = K = 8192
+    rs = np.random.RandomState(1)
+    A = torch.from_numpy(rs.uniform(-1, 1, (M, K)).astype(np.float32)).cuda()
+    B = torch.from_numpy(rs.uniform(-1, 1, (K, N)).astype(np.float32)).cuda()
+    Cm = torch.empty((M, N), device="cuda")
+    flops = 2.0 * M * N * K
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for mode in ("fp32", "bf16x3"):
+        plain = float(np.median(bench_gemm(cc, mode, A, B, Cm, 1.0, 0.0, args.runs, args.warmup,
+                                           not args.no_flush)))
+        fused = float(np.median(bench_gemm(cc, mode, A, B, Cm, 1.0, 0.0, args.runs, args.warmup,
+                                           not args.no_flush, epilogue="sigmoid")))
+        # separate pass: the GEMM, then an element-wise sigmoid over C
+        for _ in range(3):
+            cc.gemm(A, B, Cm, mode=mode)
+            torch.sigmoid_(Cm)
+        torch.cuda.synchronize()
+        reps = max(3, args.runs // 5)
+        ev[0].record()
+        for _ in range(reps):
+            cc.gemm(A, B, Cm, mode=mode)
+            torch.sigmoid_(Cm)
+        ev[1].record()
+        torch.cuda.synchronize()
+        separate = ev[0].elapsed_time(ev[1]) / reps
+        emit(dict(config="epilogue", mode=mode, M=M, N=N, K=K, act="sigmoid", ms_plain=plain,
+                  ms_fused=fused, ms_separate_pass=separate,
+                  tflops_fused=flops / fused / 1e9, fused_overhead_pct=100 * (fused / plain - 1),
+                  separate_overhead_pct=100 * (separate / plain - 1)))
 
 
 def cfg_dgemm(cc, torch, args, emit):
@@ -264,7 +307,7 @@ def cfg_autograd(cc, torch, args, emit):
 
 
 CONFIGS = dict(plain=cfg_plain, square=cfg_square, flat=cfg_flat, dgemm=cfg_dgemm,
-               grouped=cfg_grouped, autograd=cfg_autograd)
+               grouped=cfg_grouped, autograd=cfg_autograd, epilogue=cfg_epilogue)
 
 
 def main(argv=None) -> int:

@@ -56,6 +56,14 @@ extern "C" {
 #define COSMA_MODE_BF16X3 2 /* FP32 in, bf16 hi/lo split, 3 bf16 MFMA products, FP32 out */
 #define COSMA_MODE_BF16X4 3 /* FP32 in, bf16 hi/lo split, 4 products (adds lo*lo), FP32 out */
 
+/* Fused element-wise epilogues: C = act(alpha * op(A) op(B) + beta * C).
+ * Replace the ReLU / sigmoid loops of store_C (Kernel/cuda/src/cuCOSMAV100.cuh:2466-2476)
+ * and the separate sigmoid pass (Kernel/cuda/src/sigmoid4.cuh:13-27).  With split-K
+ * the activation runs once, after the fixed-order reduction. */
+#define COSMA_EPILOGUE_NONE 0
+#define COSMA_EPILOGUE_RELU 1    /* fmax(x, 0); NaN -> 0 */
+#define COSMA_EPILOGUE_SIGMOID 2 /* 1 / (1 + exp(-x)) in the element type */
+
 /* Full description of one GEMM: C = alpha * op(A) * op(B) + beta * C with
  * op(A) M x K and op(B) K x N.  trans_a == 0: A(i,k) = a[i*lda + k];
  * trans_a == 1: A(i,k) = a[k*lda + i].  trans_b == 0: B(k,j) = b[k*ldb + j];
@@ -78,6 +86,7 @@ typedef struct cosma_gemm_desc {
     void *workspace;        /* split-K partials; NULL: library-owned buffer */
     size_t workspace_bytes; /* size of workspace */
     void *stream;           /* hipStream_t; NULL: default stream */
+    int epilogue;           /* COSMA_EPILOGUE_*: activation applied to the final C */
 } cosma_gemm_desc;
 
 /* Fill *d with defaults (heuristic split and tile, default stream). */

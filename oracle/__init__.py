@@ -60,6 +60,7 @@ def _load():
         "orc_err_f64": (None, [i, i, vp, i, vp, i, vp]),
         "orc_sumsq_f32": (d, [i64, vp]),
         "orc_sumsq_f64": (d, [i64, vp]),
+        "orc_activate_f64": (None, [i64, vp, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -236,6 +237,16 @@ def err_stats(Cm: np.ndarray, R: np.ndarray) -> dict:
         lib.orc_err_f64(R.shape[0], R.shape[1], _p(Cm), Cm.shape[1], _p(R), R.shape[1], _p(out))
     return dict(sum_sq_err=out[0], sum_sq_ref=out[1], max_abs=out[2], max_rel=out[3],
                 max_ref=out[4])
+
+
+ACTIVATIONS = {None: 0, "none": 0, "relu": 1, "sigmoid": 2}
+
+
+def activate(x, act) -> np.ndarray:
+    """ReLU / sigmoid of the fused epilogue, computed in double (copy of x)."""
+    y = np.array(x, dtype=np.float64, copy=True, order="C")
+    lib.orc_activate_f64(y.size, _p(y), ACTIVATIONS[act])
+    return y
 
 
 def sumsq(x: np.ndarray) -> float:

@@ -17,6 +17,8 @@
  *                      alpha == 0 -> C = BETA*C only    Kernel/cuda/src/cuCOSMAV100.cuh:2782-2797
  *   orc_cpu_naive_sgemm the CPU verification product      Kernel/cuda/src/Matrix.h:183-201
  *   orc_ref_fill_random Matrix<T>::fillRandom              Kernel/cuda/src/Matrix.h:75-87
+ *   orc_activate_f64    fused ReLU / sigmoid epilogue       Kernel/cuda/src/cuCOSMAV100.cuh:2466-2476,
+ *                                                           Kernel/cuda/src/sigmoid4.cuh:13-27
  *   orc_err_*           max-abs comparison                  Kernel/cuda/src/Matrix.h:129-173
  *                       (plus the normwise and max-relative metrics this
  *                       project reports)
@@ -43,6 +45,9 @@
  *   orc_bf16_*                  RNE vs torch.bfloat16 conversion; closed forms
  *   orc_gemm_bf16x{3,4}_f64     exact on 8-bit integers; bounds vs orc_gemm_f64;
  *                               dropped-term planted checks
+ *   orc_activate_f64            ReLU closed forms (negatives and NaN -> 0);
+ *                               sigmoid vs scipy.special.expit, sigmoid(0) =
+ *                               1/2, sigmoid(x) + sigmoid(-x) = 1
  *   orc_err_*                   vs numpy.linalg.norm
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off -shared).
@@ -513,4 +518,16 @@ double orc_sumsq_f64(int64_t n, const double *x) {
     double s = 0;
     for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
     return s;
+}
+
+/* Output activations of the fused epilogue, in double: act 1 is ReLU
+ * (fmax(x, 0), so NaN -> 0 as with fmaxf), act 2 the logistic sigmoid
+ * 1 / (1 + exp(-x)); act 0 leaves x unchanged.  The reference applies them to
+ * the thread tile just before the store (Kernel/cuda/src/cuCOSMAV100.cuh:2466-2476)
+ * or as a separate pass over C (Kernel/cuda/src/sigmoid4.cuh:13-27). */
+void orc_activate_f64(int64_t n, double *x, int act) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (act == 1) x[i] = fmax(x[i], 0.0);
+        else if (act == 2) x[i] = 1.0 / (1.0 + exp(-x[i]));
+    }
 }

@@ -57,8 +57,16 @@ def test_tile_table_and_heuristic():
         t = cucosma_amd.tile_configs(mode)[cfg]
         assert split == 1 and t["BM"] * t["BN"] >= 128 * 128
         assert (8192 // t["BM"]) * (8192 // t["BN"]) >= 512
+    # Deep K: the large tile, split until every CU holds one workgroup.
+    for mode in (0, 2, 3):
+        cfg, split = cucosma_amd.heuristic(mode, 256, 256, 1 << 20)
+        t = cucosma_amd.tile_configs(mode)[cfg]
+        assert (256 // t["BM"]) * (256 // t["BN"]) * split == 256
+        assert (1 << 20) // split >= 512
+    # An output smaller than the large tile keeps a 128-row tile.
+    cfg, split = cucosma_amd.heuristic(0, 128, 128, 1 << 16)
+    assert cucosma_amd.tile_configs(0)[cfg]["BM"] <= 128
     cfg, split = cucosma_amd.heuristic(0, 256, 256, 1 << 20)
-    assert 8 <= split <= 64
     assert cucosma_amd.heuristic(1, 32768, 1024, 1024)[1] == 1
     assert cucosma_amd.workspace_bytes(256, 256, 1 << 20) == split * 256 * 256 * 4
     assert cucosma_amd.workspace_bytes(256, 256, 1 << 20, split_k=1) == 0
@@ -92,6 +100,8 @@ def _desc(**kw):
         (dict(trans_a=2), -1),
         (dict(split_k=5000), -1),
         (dict(tile_cfg=99), -1),
+        (dict(epilogue=3), -1),
+        (dict(epilogue=-1), -1),
         (dict(a=0x1002), -2),
         (dict(b=0x2001), -2),
         (dict(mode=1, a=0x1004), -2),

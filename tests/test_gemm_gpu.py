@@ -189,6 +189,42 @@ def test_alpha_beta_semantics(cc, dev, mode, dt):
     assert np.array_equal(Cd.cpu().numpy(), (C0 * 2).astype(dt))
 
 
+@pytest.mark.parametrize("mode,dt", MODES)
+@pytest.mark.parametrize("act", ["relu", "sigmoid"])
+def test_fused_epilogue(cc, dev, mode, dt, act):
+    """C = act(alpha*AB + beta*C) with the activation fused into the store (or
+    run after the split-K reduction) equals the activation of the unfused
+    result: ReLU bitwise, sigmoid within 4 ulp of the oracle's double-precision
+    sigmoid of the same values.  The alpha == 0 path applies it to beta*C."""
+    rng = np.random.default_rng(300)
+    ulp = 2.0**-23 if dt == np.float32 else 2.0**-52
+    for (M, N, K, split) in ((256, 256, 128, 1), (200, 136, 300, 1), (128, 192, 1024, 3)):
+        A = rng.uniform(-1, 1, (M, K)).astype(dt)
+        B = rng.uniform(-1, 1, (K, N)).astype(dt)
+        C0 = rng.uniform(-1, 1, (M, N)).astype(dt)
+        Ad, Bd = _t(A, dev), _t(B, dev)
+        plain, fused = _t(C0, dev), _t(C0, dev)
+        cc.gemm(Ad, Bd, plain, alpha=1.5, beta=0.5, mode=mode, split_k=split)
+        cc.gemm(Ad, Bd, fused, alpha=1.5, beta=0.5, mode=mode, split_k=split, epilogue=act)
+        p, f = plain.cpu().numpy(), fused.cpu().numpy()
+        want = oracle.activate(p, act)
+        assert (p < 0).any() and (p > 0).any()
+        if act == "relu":
+            assert np.array_equal(f, want.astype(dt)), (M, N, K, split)
+        else:
+            assert np.all(np.abs(f - want) <= 4 * ulp * want), (M, N, K, split)
+    Cd = _t(C0, dev)
+    cc.gemm(Ad, Bd, Cd, alpha=0.0, beta=0.5, mode=mode, epilogue=act)
+    want = oracle.activate(C0.astype(np.float64) * 0.5, act)
+    got = Cd.cpu().numpy()
+    if act == "relu":
+        assert np.array_equal(got, want.astype(dt))
+    else:
+        assert np.all(np.abs(got - want) <= 4 * ulp * want)
+    with pytest.raises(ValueError):
+        cc.gemm(Ad, Bd, Cd, mode=mode, epilogue="tanh")
+
+
 def test_residual_stats_on_gpu_match_host(cc, dev):
     rng = np.random.default_rng(103)
     R = rng.normal(0, 1, (513, 257))

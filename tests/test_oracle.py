@@ -342,3 +342,28 @@ def test_err_stats_vs_numpy():
     assert st["max_rel"] == (np.abs(d)[nz] / np.abs(R)[nz]).max()
     A = rng.normal(0, 1, (5, 6)).astype(np.float32)
     assert np.isclose(oracle.sumsq(A), np.linalg.norm(A.astype(np.float64)) ** 2, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- epilogue
+
+def test_activation_relu_closed_forms():
+    x = np.array([-2.5, -0.0, 0.0, 3.0, np.nan, -np.inf, np.inf, 1e-300, -1e-300])
+    got = oracle.activate(x, "relu")
+    want = np.array([0.0, 0.0, 0.0, 3.0, 0.0, 0.0, np.inf, 1e-300, 0.0])
+    assert np.array_equal(got, want)
+    assert np.array_equal(oracle.activate(x[:4], None), x[:4])
+
+
+def test_activation_sigmoid_vs_library_and_identities():
+    special = pytest.importorskip("scipy.special")
+    rng = np.random.default_rng(15)
+    x = np.concatenate([rng.normal(0, 8, 4000), [0.0, 1.0, -1.0, 30.0, -30.0, 700.0, -700.0]])
+    got = oracle.activate(x, "sigmoid")
+    ref = special.expit(x)
+    assert np.allclose(got, ref, rtol=4e-16, atol=0)
+    assert oracle.activate([0.0], "sigmoid")[0] == 0.5
+    # s(x) + s(-x) = 1 and s'(0) = 1/4 (a sign or a dropped exp would fail one of these)
+    assert np.allclose(got + oracle.activate(-x, "sigmoid"), 1.0, rtol=0, atol=5e-16)
+    h = 1e-6
+    d = (oracle.activate([h], "sigmoid")[0] - oracle.activate([-h], "sigmoid")[0]) / (2 * h)
+    assert abs(d - 0.25) < 1e-9
