@@ -67,19 +67,28 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // Heuristic tile / split choice, used when neither the caller nor the tuned
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
-//  - large outputs (at least two workgroups per CU on the mode's large tile):
-//    FP32 256x128 (8 waves of 64x64, next-stage copy between the MFMAs);
-//    FP64 128x128 (8 waves of 64x32, 4 waves per SIMD, ping-pong copies);
-//    BF16x3 256x256 (8 waves of 128x64, ping-pong copies); BF16x4 256x256
-//    (copy between the MFMAs, which leaves room for its per-k-step
-//    temporaries);
-//  - deep-K outputs (few tiles, K >= 16384): split K over the large tile --
-//    FP32 256x128 and BF16x3 256x256 with ping-pong copies, BF16x4 256x256 --
-//    until every CU holds one workgroup (each K slice at least 512 deep; at
-//    256x256x1048576 that is 256 slices of 4096, so every input byte is read
-//    from HBM once); FP64, and outputs smaller than the large tile, split a
-//    128-row tile with a long K chunk (128x64x32, or 128x128x64 for BF16)
-//    until every CU holds two (128x64) or one (128x128x64) workgroups;
+//  - BF16x3 / BF16x4: the large tile -- 256x256, 8 waves of 128x64 (BF16x3
+//    with ping-pong copies; BF16x4 with the copy between the MFMAs, which
+//    leaves room for its per-k-step temporaries) -- whenever the output fills
+//    at least half of its tiles and the tiles, with K split into slices of at
+//    least 512, give at least half the CUs a workgroup.  K is split only when
+//    the tiles alone leave more than half the CUs idle, until every CU holds
+//    one workgroup (256x256x1048576: 256 slices of 4096, so every input byte
+//    is read from HBM once).  It beat every smaller tile from 2048^3 up, and
+//    for 4096x4096x16384 (the weight gradient of a 16384x4096x4096 layer) ran
+//    at twice the speed of 128x128;
+//  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
+//    (8 waves of 64x64, next stage held in registers; at 8192^3 it matched
+//    the copy-between-MFMAs variant with A row-major and beat it by 12% with
+//    B stored transposed), FP64 128x128 (8 waves of 64x32, 4 waves per SIMD,
+//    ping-pong copies).  FP32 splits K in two when
+//    the last round of workgroups would be at most half full (6144^3: 4.5
+//    rounds of 256 become 9; +7%);
+//  - deep-K outputs (few tiles, K >= 16384): FP32 splits K over 256x128 with
+//    ping-pong copies until every CU holds one workgroup; FP64, and outputs
+//    smaller than the large tile, split a 128-row tile with a long K chunk
+//    (128x64x32, or 128x128x64 for BF16) until every CU holds two (128x64) or
+//    one (128x128x64) workgroups;
 //  - otherwise 128x128 (BF16) / 128x64 tiles while they give every CU a
 //    workgroup, else 64x64; split K when the 64x64 grid still leaves CUs idle
 //    (FP32/FP64) or gives them fewer than two workgroups (BF16), keeping every
@@ -88,7 +97,7 @@ struct Choice {
   int cfg, split;
 };
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
-constexpr int large_cfg(int mode) { return mode == 0 ? 3 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
+constexpr int large_cfg(int mode) { return mode == 0 ? 5 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
 // with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
@@ -105,7 +114,16 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
 }
 constexpr Choice heuristic(int mode, int m, int n, int k) {
   int cfg = large_cfg(mode);
-  if (tiles_of(mode, cfg, m, n) >= 2 * kCUs) return Choice{cfg, 1};
+  const int lt = tiles_of(mode, cfg, m, n);
+  if (split_mode(mode)) {
+    const int64_t covered = (int64_t)lt * kDims[mode][cfg].bm * kDims[mode][cfg].bn;
+    const int split = lt >= kCUs / 2 ? 1 : clamp_split(kCUs / lt, k, 512, kCUs);
+    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2) return Choice{cfg, split};
+  } else if (lt >= 2 * kCUs) {
+    const int rem = lt % kCUs;
+    const bool even_out = mode == 0 && rem > 0 && rem <= kCUs / 2 && lt < 8 * kCUs && k >= 2048;
+    return Choice{cfg, even_out ? 2 : 1};
+  }
   const int dc = deep_cfg(mode, m, n);
   const int dtiles = tiles_of(mode, dc, m, n);
   if (k >= 16384 && dtiles < kCUs) {

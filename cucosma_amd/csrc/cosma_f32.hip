@@ -5,10 +5,11 @@
 // cfg 0: 128x128, 4 waves of 64x64
 // cfg 1: 64x64, 4 waves of 32x32 (small shapes, split-K slices)
 // cfg 2: 128x64, 4 waves of 64x32 (deep-K split)
-// cfg 3: 256x128, 8 waves of 64x64 (large shapes: 134 TFLOP/s at 8192^3)
+// cfg 3: 256x128, 8 waves of 64x64
 // cfg 4: 128x128 with a 64-deep K chunk
-// cfg 5: cfg 3 with register double-buffered prefetch (PIPE 3)
-// cfg 6: cfg 3 with ping-pong copies (PIPE 4)
+// cfg 5: cfg 3 with register double-buffered prefetch (PIPE 3) (large
+//        shapes: 135 TFLOP/s at 8192^3, 141 with B stored transposed)
+// cfg 6: cfg 3 with ping-pong copies (PIPE 4) (deep-K split)
 #include "cosma_launch.hpp"
 
 namespace cosma {
