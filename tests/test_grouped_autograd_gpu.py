@@ -90,6 +90,30 @@ def test_grouped_empty_and_explicit_tiles(cc, dev):
     assert torch.all(Cm == 32)
 
 
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16x4", "fp64"])
+def test_grouped_bitwise_equals_single_launches(cc, dev, mode):
+    """Every tile configuration in one plan: each problem's result is bitwise
+    the single-launch result of the same tile configuration without split-K
+    (the grouped kernel runs the same per-tile body)."""
+    dt = torch.float64 if mode == "fp64" else torch.float32
+    ncfg = len(cc.tile_configs(cc._lib.MODES[mode]))
+    sizes = _sizes(5, 2 * ncfg) + [(33, 17, 5), (7, 300, 64), (300, 7, 129)]
+    cfgs = [i % ncfg for i in range(len(sizes))]
+    g = torch.Generator(device="cpu").manual_seed(5)
+    As = [(torch.rand(m, k, generator=g, dtype=dt) * 2 - 1).to(dev) for m, n, k in sizes]
+    Bs = [(torch.rand(k, n, generator=g, dtype=dt) * 2 - 1).to(dev) for m, n, k in sizes]
+    C0s = [(torch.rand(m, n, generator=g, dtype=dt) * 2 - 1).to(dev) for m, n, k in sizes]
+    Cs = [c.clone() for c in C0s]
+    plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, alpha=1.5, beta=0.5, tile_cfgs=cfgs)
+    assert {b["tile_cfg"] for b in plan.info()["buckets"]} == set(range(ncfg))
+    plan.run()
+    for A, B, C0, Cg, cfg in zip(As, Bs, C0s, Cs, cfgs):
+        Cs1 = C0.clone()
+        cc.gemm(A, B, Cs1, alpha=1.5, beta=0.5, mode=mode, tile_cfg=cfg, split_k=1)
+        assert torch.equal(Cg, Cs1), (tuple(A.shape), tuple(B.shape), cfg)
+    plan.close()
+
+
 def test_autograd_fp64_gradcheck(cc, dev):
     X = torch.randn(12, 20, dtype=torch.float64, device=dev, requires_grad=True)
     W = torch.randn(20, 9, dtype=torch.float64, device=dev, requires_grad=True)
