@@ -221,7 +221,8 @@ def cfg_grouped(cc, torch, args, emit):
         edges = (256, 1024)
         bucket = [0 if max(m, n) < edges[0] else (1 if max(m, n) < edges[1] else 2)
                   for m, n, k in sizes]
-        cand = (1, 0) if mode == "fp32" else (1, 0, 2)
+        large = {"fp32": 5, "bf16x3": 6}[mode]
+        cand = (1, 0, large)
         best = None
         tried = []
         for c0 in cand:
@@ -266,6 +267,7 @@ def cfg_grouped(cc, torch, args, emit):
         info = plan.info()
         emit(dict(config="grouped", mode=mode, problems=len(sizes), gflop=flops / 1e9,
                   bucket_edges=list(edges), bucket_tile_cfgs=list(best[0]), buckets=info["buckets"],
+                  searched={",".join(map(str, c)): round(t * 1e3, 4) for c, t in tried},
                   ms_graph=ms_graph, tflops_graph=flops / ms_graph / 1e9,
                   ms_per_problem_launches=ms_loop, tflops_per_problem_launches=flops / ms_loop / 1e9,
                   speedup=ms_loop / ms_graph,
