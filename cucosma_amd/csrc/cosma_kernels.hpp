@@ -20,12 +20,12 @@
 // through a leading dimension.  TA == false: A(i,k) = A[i*lda + k] (row-major,
 // the reference's only layout); TA == true: A(i,k) = A[k*lda + i].  TB ==
 // false: B(k,j) = B[k*ldb + j]; TB == true: B(k,j) = B[j*ldb + k].  In LDS both
-// operands are stored "k-contiguous": sA[i][k], sB[j][k], rows padded by 16
-// bytes so that the 16-byte fragment reads of 8 consecutive rows fall into
-// distinct bank groups.  An operand whose global layout is k-contiguous is
-// copied with 16-byte loads and stores; the other one is loaded as 4x4 (2x2
-// for FP64) element blocks and transposed in registers before its 16-byte LDS
-// stores.
+// operands are stored "k-contiguous": sA[i][k], sB[j][k], rows of BK elements
+// without padding, the 16-byte chunks of each row permuted (LdsLayout) so
+// that fragment reads and tile stores are free of bank conflicts.  An operand
+// whose global layout is k-contiguous is copied with 16-byte loads and
+// stores; the other one is loaded as 4x4 (2x2 for FP64) element blocks and
+// transposed in registers before its LDS stores.
 //
 // MFMA operand mapping (16x16 output sub-tile, 64 lanes, g = lane/16,
 // r16 = lane%16): lane supplies A(row r16, k-group g) and B(k-group g, col
@@ -69,7 +69,6 @@ struct Traits<kF32> {
   static constexpr int GVEC = 4;  // elements per global vector
   static constexpr int KPL = 4;   // consecutive k per lane per fragment read
   static constexpr int STEPS = 4; // MFMAs per fragment read
-  static constexpr int PAD = 4;   // LDS row padding (elements) = 16 bytes
   static constexpr int PLANES = 1;
 };
 template <>
@@ -83,7 +82,6 @@ struct Traits<kF64> {
   static constexpr int GVEC = 2;
   static constexpr int KPL = 2;
   static constexpr int STEPS = 2;
-  static constexpr int PAD = 2;
   static constexpr int PLANES = 1;
 };
 template <>
@@ -97,7 +95,6 @@ struct Traits<kBF16x3> {
   static constexpr int GVEC = 4;
   static constexpr int KPL = 8;
   static constexpr int STEPS = 1;
-  static constexpr int PAD = 8;
   static constexpr int PLANES = 2;  // hi and lo
 };
 // Four products: the three of kBF16x3 plus lo*lo.
@@ -138,6 +135,45 @@ __device__ __forceinline__ E activate(E v, int act) {
   if (act == kActSigmoid) return E(1) / (E(1) + exp(-v));
   return v;
 }
+
+// LDS placement of one operand tile: row r (M or N index) holds BK elements
+// in NC 16-byte chunks, chunk c of row r stored at 16-byte unit phys(r, c).
+// gfx950's LDS has 64 banks of 4 bytes; a 16-byte read by 64 lanes is served
+// 16 lanes at a time, lanes {0-7, 16-23}, {8-15, 24-31}, {32-39, 48-55},
+// {40-47, 56-63} (measured with SQ_LDS_BANK_CONFLICT: padded rows of 80 or
+// 144 bytes took 8 cycles per read instead of 4).  A fragment read puts rows
+// r0..r0+7 and two adjacent chunks in one such group, so the 16 chunks must
+// land in 16 distinct bank quads:
+//   NC = 16 (256-byte rows, every row starts at bank 0): c ^ 2*(r & 7);
+//   NC = 8 (two rows per 64 banks): c ^ 2*((r >> 1) & 3);
+//   NC = 4 (four rows per 64 banks): c ^ 2*((r >> 2) & 1), and rows r and r+4
+//   of a block of four rows also swap bank quarters ((r & 3) ^ ((r >> 2) & 3))
+//   -- without that, the transposing store, which writes rows 4j + t for
+//   eight values of j in one instruction, puts all of them in one quarter.
+// Stores of 8 or 16 bytes per lane are conflict-free in every case (measured
+// on the loaders' exact lane patterns).  phys maps each block of four rows
+// (NC = 4) or each row (NC = 8, 16) onto its own units, so a tile of R rows
+// (R a multiple of 4) occupies exactly R * BK elements.
+template <int MODE, int BK>
+struct LdsLayout {
+  using S = typename Traits<MODE>::S;
+  static constexpr int EPC = 16 / (int)sizeof(S);  // elements per 16-byte chunk
+  static constexpr int NC = BK / EPC;              // chunks per row
+  static_assert(NC == 4 || NC == 8 || NC == 16, "LDS rows of 64, 128 or 256 bytes");
+  // Element offset of element e (of a run of at most EPC - e % EPC
+  // contiguous elements) of row r.
+  __device__ __forceinline__ static int off(int r, int e) {
+    const int c = e / EPC;
+    int u;
+    if constexpr (NC == 4)
+      u = ((r >> 2) << 4) + (((r & 3) ^ ((r >> 2) & 3)) << 2) + (c ^ (((r >> 2) & 1) << 1));
+    else if constexpr (NC == 8)
+      u = (r << 3) + (c ^ (((r >> 1) & 3) << 1));
+    else
+      u = (r << 4) + (c ^ ((r & 7) << 1));
+    return u * EPC + (e % EPC);
+  }
+};
 
 // Split four floats into RNE bf16 hi parts and RNE bf16 lo = x - hi parts
 // (x - hi is exact in FP32).  v_cvt_pk_bf16_f32 does the rounding.
@@ -187,7 +223,7 @@ struct TileLoader {
   using GV = typename T::GV;
   using S = typename T::S;
   static constexpr int GVEC = T::GVEC;
-  static constexpr int LDK = BK + T::PAD;
+  using L = LdsLayout<MODE, BK>;
   // KMAJ: units are 16-byte vectors along k.  Otherwise units are GVEC x GVEC
   // blocks (GVEC k rows of one GVEC-wide vector along the row index); the k
   // block varies fastest across lanes, so the transposed LDS stores of
@@ -271,7 +307,7 @@ struct TileLoader {
         if (!active(idx)) continue;
         const int r = idx / VPR;
         const int kv = idx - r * VPR;
-        put(sh, sl, r * LDK + kv * GVEC, v[i]);
+        put(sh, sl, L::off(r, kv * GVEC), v[i]);
       }
     } else {
       constexpr int KB = BK / GVEC;
@@ -286,7 +322,7 @@ struct TileLoader {
           GV w;
 #pragma unroll
           for (int u = 0; u < GVEC; ++u) w[u] = v[i * GVEC + u][t];
-          put(sh, sl, (rb * GVEC + t) * LDK + kb * GVEC, w);
+          put(sh, sl, L::off(rb * GVEC + t, kb * GVEC), w);
         }
       }
     }
@@ -331,9 +367,8 @@ struct TileShape {
   static constexpr int NT = 64 * WAVES_M * WAVES_N;
   static constexpr int TM = WM / 16;
   static constexpr int TN = WN / 16;
-  static constexpr int LDK = BK + Traits<MODE>::PAD;
   static constexpr int KR = 4 * Traits<MODE>::KPL;  // k covered per fragment read round
-  static constexpr int PLANE = (BM + BN) * LDK;      // elements of one hi or lo plane
+  static constexpr int PLANE = (BM + BN) * BK;       // elements of one hi or lo plane
   static constexpr int STAGE = PLANE * Traits<MODE>::PLANES;
   static_assert(BM % WM == 0 && BN % WN == 0 && WM % 16 == 0 && WN % 16 == 0, "wave tiling");
   static_assert(BK % KR == 0, "BK must cover whole fragment rounds");
@@ -366,13 +401,14 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
   using T = Traits<MODE>;
   using Frag = typename T::Frag;
   using Acc = typename T::Acc;
-  constexpr int TM = Sh::TM, TN = Sh::TN, LDK = Sh::LDK;
+  constexpr int TM = Sh::TM, TN = Sh::TN;
+  using L = LdsLayout<MODE, BK>;
   constexpr int NQ = BK / Sh::KR;
   constexpr int kMidRow = PIPE == 1 ? TM / 2 : (PIPE == 2 ? (3 * TM) / 4 : TM / 4);  // row before which mid() runs
   const int g = lane >> 4;
   const int r16 = lane & 15;
   const typename T::S* sa = st;
-  const typename T::S* sb = st + BM * LDK;
+  const typename T::S* sb = st + BM * BK;
 
   // One sub-tile row of the split modes: smallest terms first, then the
   // leading hi*hi term, the product index outermost so TN independent MFMAs
@@ -420,11 +456,11 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
     const int kof = q * Sh::KR + g * T::KPL;
     if constexpr (is_split(MODE)) {
       const typename T::S* sal = st + Sh::PLANE;
-      const typename T::S* sbl = sal + BM * LDK;
+      const typename T::S* sbl = sal + BM * BK;
       Frag bh[TN], bl[TN];
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int off = (wn0 + j * 16 + r16) * LDK + kof;
+        const int off = L::off(wn0 + j * 16 + r16, kof);
         bh[j] = *reinterpret_cast<const Frag*>(sb + off);
         bl[j] = *reinterpret_cast<const Frag*>(sbl + off);
       }
@@ -432,7 +468,7 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
         Frag ah[TM], al[TM];
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
-          const int off = (wm0 + i * 16 + r16) * LDK + kof;
+          const int off = L::off(wm0 + i * 16 + r16, kof);
           ah[i] = *reinterpret_cast<const Frag*>(sa + off);
           al[i] = *reinterpret_cast<const Frag*>(sal + off);
         }
@@ -442,7 +478,7 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
           if (q == NQ - 1 && i == kMidRow) mid();
-          const int off = (wm0 + i * 16 + r16) * LDK + kof;
+          const int off = L::off(wm0 + i * 16 + r16, kof);
           const Frag ah = *reinterpret_cast<const Frag*>(sa + off);
           const Frag al = *reinterpret_cast<const Frag*>(sal + off);
           split_row(acc[i], ah, al, bh, bl);
@@ -452,12 +488,12 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
       Frag b[TN];
 #pragma unroll
       for (int j = 0; j < TN; ++j)
-        b[j] = *reinterpret_cast<const Frag*>(sb + (wn0 + j * 16 + r16) * LDK + kof);
+        b[j] = *reinterpret_cast<const Frag*>(sb + L::off(wn0 + j * 16 + r16, kof));
       if constexpr (PIPE == 0) {
         Frag a[TM];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
-          a[i] = *reinterpret_cast<const Frag*>(sa + (wm0 + i * 16 + r16) * LDK + kof);
+          a[i] = *reinterpret_cast<const Frag*>(sa + L::off(wm0 + i * 16 + r16, kof));
 #pragma unroll
         for (int s = 0; s < T::STEPS; ++s)
 #pragma unroll
@@ -468,7 +504,7 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
           if (q == NQ - 1 && i == kMidRow) mid();
-          const Frag a = *reinterpret_cast<const Frag*>(sa + (wm0 + i * 16 + r16) * LDK + kof);
+          const Frag a = *reinterpret_cast<const Frag*>(sa + L::off(wm0 + i * 16 + r16, kof));
 #pragma unroll
           for (int s = 0; s < T::STEPS; ++s)
 #pragma unroll
@@ -529,7 +565,6 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
   using T = Traits<MODE>;
   using S = typename T::S;
   constexpr int NT = Sh::NT;
-  constexpr int LDK = Sh::LDK;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -558,7 +593,7 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
 
   auto store_to = [&](const auto& a, const auto& b, S* st) {
     a.store(st, st + Sh::PLANE, tid);
-    b.store(st + BM * LDK, st + Sh::PLANE + BM * LDK, tid);
+    b.store(st + BM * BK, st + Sh::PLANE + BM * BK, tid);
   };
 
   if constexpr (PIPE == 3) {
