@@ -26,6 +26,7 @@ from .gemm import (  # noqa: F401
     sgemm,
     sgemm_bf16x3,
     sgemm_bf16x4,
+    split_operand,
     sumsq,
     to_f64,
     workspace_bytes,

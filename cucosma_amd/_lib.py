@@ -64,6 +64,8 @@ class GemmDesc(C.Structure):
         ("workspace_bytes", C.c_size_t),
         ("stream", C.c_void_p),
         ("epilogue", C.c_int),
+        ("a_planes", C.c_void_p),
+        ("b_planes", C.c_void_p),
     ]
 
 
@@ -139,6 +141,8 @@ def _load():
         "cosma_tune_set": (i, [i] * 9),
         "cosma_tune_lookup": (i, [i] * 6 + [C.POINTER(i)] * 3),
         "cosma_tune_clear": (i, []),
+        "cosma_planes_bytes": (C.c_size_t, [i, i]),
+        "cosma_split_operand": (i, [i, i, i, i, vp, i64, vp, vp]),
         "cosma_status_string": (C.c_char_p, [i]),
         "cosma_version": (C.c_char_p, []),
     }

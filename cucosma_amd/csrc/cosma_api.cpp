@@ -49,15 +49,15 @@ constexpr TileDims kDims[4][kMaxCfg] = {
     // FP64
     {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 16}, {64, 64, 32}, {128, 128, 32},
      {256, 128, 16}, {128, 128, 16}},
-    // BF16X3
+    // BF16X3 (cfg 7: pre-split operands)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}},
+     {256, 256, 32}, {256, 256, 32}},
     // BF16X4 (same tiles as BF16X3)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}},
+     {256, 256, 32}, {256, 256, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {7, 8, 7, 7};
+constexpr int kNumCfg[kNumModes] = {7, 8, 8, 8};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -77,7 +77,11 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    one workgroup (256x256x1048576: 256 slices of 4096, so every input byte
 //    is read from HBM once).  It beat every smaller tile from 2048^3 up, and
 //    for 4096x4096x16384 (the weight gradient of a 16384x4096x4096 layer) ran
-//    at twice the speed of 128x128;
+//    at twice the speed of 128x128.  When M*N/(M+N) >= 3072 (8192^3,
+//    16384x4096, 12288^3, ...) the same tile runs over pre-split operands
+//    (cfg 7): the split pass costs (M+N)*K*8 bytes of HBM traffic, and the
+//    GEMM without the in-loop split is 16% faster (8192^3: 2.48 vs 2.64 ms in
+//    total; 4096^3 and 2048x8192x2048 lose 7-15%, so they keep cfg 6);
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    (8 waves of 64x64, next stage held in registers; at 8192^3 it matched
 //    the copy-between-MFMAs variant with A row-major and beat it by 12% with
@@ -98,6 +102,9 @@ struct Choice {
   int cfg, split;
 };
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
+// The tile configuration over pre-split operands (no grouped launch).
+constexpr int kPlanarCfg = 7;
+constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && cfg == kPlanarCfg; }
 constexpr int large_cfg(int mode) { return mode == 0 ? 5 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
@@ -119,7 +126,11 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
   if (split_mode(mode)) {
     const int64_t covered = (int64_t)lt * kDims[mode][cfg].bm * kDims[mode][cfg].bn;
     const int split = lt >= kCUs / 2 ? 1 : clamp_split(kCUs / lt, k, 512, kCUs);
-    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2) return Choice{cfg, split};
+    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2) {
+      // Large outputs: split the operands once and copy planes into LDS.
+      if (split == 1 && lt >= kCUs && (int64_t)m * n >= 3072 * ((int64_t)m + n)) return Choice{kPlanarCfg, 1};
+      return Choice{cfg, split};
+    }
   } else if (lt >= 2 * kCUs) {
     const int rem = lt % kCUs;
     const bool even_out = mode == 0 && rem > 0 && rem <= kCUs / 2 && lt < 8 * kCUs && k >= 2048;
@@ -141,6 +152,19 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
 }
 
 size_t elem_size(int mode) { return mode == COSMA_MODE_FP64 ? 8 : 4; }
+
+// Workspace of one call: split-K partials [split][M][N] at offset 0, then (for
+// a pre-split configuration) room for the planes of the operands the caller
+// did not split, 256-byte aligned.
+size_t workspace_need(const cosma_gemm_desc* d, int cfg, int split, size_t* planes_off) {
+  size_t bytes = split > 1 ? (size_t)split * (size_t)d->m * (size_t)d->n * elem_size(d->mode) : 0;
+  *planes_off = (bytes + 255) / 256 * 256;
+  if (is_planar(d->mode, cfg)) {
+    const int rows_a = d->a_planes ? 0 : d->m, rows_b = d->b_planes ? 0 : d->n;
+    if (rows_a + rows_b > 0) bytes = *planes_off + cosma::planar_bytes(rows_a, rows_b, d->k);
+  }
+  return bytes;
+}
 
 std::mutex g_mu;
 struct DevState {
@@ -234,6 +258,11 @@ int validate(const cosma_gemm_desc* d) {
   if (d->tile_cfg >= kNumCfg[d->mode]) return COSMA_STATUS_INVALID_VALUE;
   if (d->epilogue < COSMA_EPILOGUE_NONE || d->epilogue > COSMA_EPILOGUE_SIGMOID)
     return COSMA_STATUS_INVALID_VALUE;
+  if (d->a_planes || d->b_planes) {
+    if (!split_mode(d->mode)) return COSMA_STATUS_INVALID_VALUE;
+    if (d->tile_cfg >= 0 && !is_planar(d->mode, d->tile_cfg)) return COSMA_STATUS_INVALID_VALUE;
+    if (((uintptr_t)d->a_planes % 16) || ((uintptr_t)d->b_planes % 16)) return COSMA_STATUS_MISALIGNED;
+  }
   if (d->m == 0 || d->n == 0) return 0;
   if (!d->c) return COSMA_STATUS_INVALID_VALUE;
   if (d->ldc < (d->n > 1 ? d->n : 1)) return COSMA_STATUS_INVALID_VALUE;
@@ -248,6 +277,24 @@ int validate(const cosma_gemm_desc* d) {
   const uintptr_t es = elem_size(d->mode);
   if (((uintptr_t)d->a % es) || ((uintptr_t)d->b % es)) return COSMA_STATUS_MISALIGNED;
   return 0;
+}
+
+// Tile configuration, split factor and group size of a validated call with
+// K > 0: the caller's, else the tuned table's, else the heuristic's.
+void resolve_choice(const cosma_gemm_desc* d, int* cfg, int* split, int* group_m) {
+  Choice ch = heuristic(d->mode, d->m, d->n, d->k);
+  *group_m = d->group_m;
+  TuneVal tv;
+  if (d->tile_cfg < 0 && d->split_k <= 0 &&
+      tuned(d->mode, d->m, d->n, d->k, d->trans_a, d->trans_b, &tv)) {
+    ch.cfg = tv.cfg;
+    ch.split = tv.split;
+    if (*group_m <= 0) *group_m = tv.group_m;
+  }
+  *cfg = d->tile_cfg >= 0 ? d->tile_cfg : ch.cfg;
+  if ((d->a_planes || d->b_planes) && !is_planar(d->mode, *cfg)) *cfg = kPlanarCfg;
+  *split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
+  if (*split > d->k) *split = d->k > 0 ? d->k : 1;
 }
 
 int run(const cosma_gemm_desc* d) {
@@ -265,15 +312,8 @@ int run(const cosma_gemm_desc* d) {
                                               (float)d->beta, d->epilogue, s);
     return (int)e;
   }
-  Choice ch = heuristic(d->mode, d->m, d->n, d->k);
-  int group_m = d->group_m;
-  TuneVal tv;
-  if (d->tile_cfg < 0 && d->split_k <= 0 &&
-      tuned(d->mode, d->m, d->n, d->k, d->trans_a, d->trans_b, &tv)) {
-    ch.cfg = tv.cfg;
-    ch.split = tv.split;
-    if (group_m <= 0) group_m = tv.group_m;
-  }
+  int cfg, split, group_m;
+  resolve_choice(d, &cfg, &split, &group_m);
   Problem p;
   p.mode = d->mode;
   p.M = d->m;
@@ -289,17 +329,23 @@ int run(const cosma_gemm_desc* d) {
   p.ldc = d->ldc;
   p.alpha = d->alpha;
   p.beta = d->beta;
-  p.cfg = d->tile_cfg >= 0 ? d->tile_cfg : ch.cfg;
-  p.split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
-  if (p.split > d->k) p.split = d->k > 0 ? d->k : 1;
+  p.cfg = cfg;
+  p.split = split;
   p.group_m = group_m;
   p.act = d->epilogue;
   p.ws = nullptr;
+  p.planes = nullptr;
+  p.a_planes = d->a_planes;
+  p.b_planes = d->b_planes;
   p.stream = s;
-  if (p.split > 1) {
-    const size_t need = (size_t)p.split * (size_t)d->m * (size_t)d->n * elem_size(d->mode);
-    st = resolve_workspace(d, need, s, &p.ws);
+  size_t planes_off = 0;
+  const size_t need = workspace_need(d, cfg, split, &planes_off);
+  if (need > 0) {
+    void* ws = nullptr;
+    st = resolve_workspace(d, need, s, &ws);
     if (st != 0) return st;
+    if (split > 1) p.ws = ws;
+    if (is_planar(d->mode, cfg)) p.planes = static_cast<char*>(ws) + planes_off;
   }
   hipError_t e;
   switch (d->mode) {
@@ -492,6 +538,7 @@ int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* pr
     if (q.k > 0 && q.alpha != 0.0 && ((uintptr_t)q.c == 0)) return COSMA_STATUS_INVALID_VALUE;
     int cfg = (tile_cfgs && tile_cfgs[i] >= 0) ? tile_cfgs[i] : group_cfg(mode, q.m, q.n);
     if (cfg >= kNumCfg[mode]) return COSMA_STATUS_INVALID_VALUE;
+    if (is_planar(mode, cfg)) return COSMA_STATUS_NOT_SUPPORTED;
     by_cfg[cfg].push_back(&q);
     flops += 2.0 * q.m * q.n * (double)q.k;
   }
@@ -617,6 +664,8 @@ void cosma_gemm_desc_init(cosma_gemm_desc* d) {
   d->workspace_bytes = 0;
   d->stream = nullptr;
   d->epilogue = COSMA_EPILOGUE_NONE;
+  d->a_planes = nullptr;
+  d->b_planes = nullptr;
 }
 
 int cosma_gemm_ex(const cosma_gemm_desc* d) { return run(d); }
@@ -627,15 +676,28 @@ int cosma_workspace_query(const cosma_gemm_desc* d, size_t* bytes) {
   if (!bytes) return COSMA_STATUS_INVALID_VALUE;
   *bytes = 0;
   if (d->m == 0 || d->n == 0 || d->k == 0 || d->alpha == 0.0) return 0;
-  Choice ch = heuristic(d->mode, d->m, d->n, d->k);
-  TuneVal tv;
-  if (d->tile_cfg < 0 && d->split_k <= 0 &&
-      tuned(d->mode, d->m, d->n, d->k, d->trans_a, d->trans_b, &tv))
-    ch.split = tv.split;
-  int split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
-  if (split > d->k) split = d->k;
-  if (split > 1) *bytes = (size_t)split * (size_t)d->m * (size_t)d->n * elem_size(d->mode);
+  int cfg, split, group_m;
+  resolve_choice(d, &cfg, &split, &group_m);
+  size_t planes_off = 0;
+  *bytes = workspace_need(d, cfg, split, &planes_off);
   return 0;
+}
+
+size_t cosma_planes_bytes(int rows, int k) {
+  if (rows <= 0 || k <= 0) return 0;
+  return cosma::planar_bytes(rows, 0, k);
+}
+
+int cosma_split_operand(int mode, int kmajor, int rows, int k, const float* X, int64_t ld, void* planes,
+                        void* stream) {
+  if (!split_mode(mode) || rows < 0 || k < 0) return COSMA_STATUS_INVALID_VALUE;
+  if (rows == 0 || k == 0) return 0;
+  if (!X || !planes) return COSMA_STATUS_INVALID_VALUE;
+  const int64_t cols = kmajor ? k : rows;
+  if (ld < cols) return COSMA_STATUS_INVALID_VALUE;
+  if (((uintptr_t)X % 4) || ((uintptr_t)planes % 16)) return COSMA_STATUS_MISALIGNED;
+  return (int)cosma::run_split_operand(mode, kmajor != 0, rows, k, X, ld, planes,
+                                       static_cast<hipStream_t>(stream));
 }
 
 int cosma_workspace_set(void* ptr, size_t bytes) {

@@ -8,6 +8,8 @@
 // cfg 4: 256x128, 4 waves of 128x64
 // cfg 5: 128x128 with a 64-deep K chunk (deep-K split)
 // cfg 6: cfg 3 with ping-pong copies (PIPE 4): the large-shape tile
+// cfg 7: 256x256, 8 waves of 128x64 over pre-split operands (split pass, then
+//        direct-to-LDS loads; Planar in cosma_launch.hpp); not for grouped plans
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -19,9 +21,10 @@ using B3C3 = Tile<kBF16x3, 256, 256, 32, 128, 64>;
 using B3C4 = Tile<kBF16x3, 256, 128, 32, 128, 64>;
 using B3C5 = Tile<kBF16x3, 128, 128, 64, 64, 64>;
 using B3C6 = Tile<kBF16x3, 256, 256, 32, 128, 64, 4>;
+using B3C7 = Planar<kBF16x3, 256, 256, 128, 64>;
 
-extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info(), B3C6::info()};
-extern const int kBF16x3NumTiles = 7;
+extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info(), B3C6::info(), B3C7::info()};
+extern const int kBF16x3NumTiles = 8;
 
 hipError_t run_gemm_bf16x3(const Problem& p) {
   switch (p.cfg) {
@@ -32,6 +35,7 @@ hipError_t run_gemm_bf16x3(const Problem& p) {
     case 4: return B3C4::run(p);
     case 5: return B3C5::run(p);
     case 6: return B3C6::run(p);
+    case 7: return B3C7::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -46,6 +50,7 @@ hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, 
     case 4: return B3C4::grouped(table, count, total, blocks, s);
     case 5: return B3C5::grouped(table, count, total, blocks, s);
     case 6: return B3C6::grouped(table, count, total, blocks, s);
+    case 7: return hipErrorNotSupported;  // pre-split: no grouped launch
     default: return hipErrorInvalidValue;
   }
 }
@@ -72,8 +77,16 @@ int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge) {
     case 4: return B3C4::occupancy(ta, tb, edge);
     case 5: return B3C5::occupancy(ta, tb, edge);
     case 6: return B3C6::occupancy(ta, tb, edge);
+    case 7: return B3C7::occupancy(ta, tb, edge);
     default: return 0;
   }
+}
+
+hipError_t run_split_operand(int mode, bool kmajor, int rows, int K, const float* X, int64_t ld, void* planes,
+                             hipStream_t s) {
+  (void)mode;  // the split is the same for three and four products
+  const int Kp = (int)(((int64_t)K + 31) / 32 * 32);
+  return B3C7::split(X, ld, kmajor, rows, K, Kp, static_cast<__bf16*>(planes), s);
 }
 
 }  // namespace cosma

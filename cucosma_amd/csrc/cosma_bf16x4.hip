@@ -3,7 +3,8 @@
 // accumulation): the tiles of cosma_bf16x3.hip; cfg 3 places the next-stage
 // copy between the MFMAs (PIPE 1), which leaves room for the per-k-step
 // temporaries without spilling; cfg 6 (ping-pong) spills and is kept only
-// for completeness of the table.
+// for completeness of the table; cfg 7 is the pre-split 256x256 tile (as in
+// cosma_bf16x3.hip).
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -15,9 +16,10 @@ using B4C3 = Tile<kBF16x4, 256, 256, 32, 128, 64, 1>;
 using B4C4 = Tile<kBF16x4, 256, 128, 32, 128, 64>;
 using B4C5 = Tile<kBF16x4, 128, 128, 64, 64, 64>;
 using B4C6 = Tile<kBF16x4, 256, 256, 32, 128, 64, 4>;
+using B4C7 = Planar<kBF16x4, 256, 256, 128, 64>;
 
-extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info(), B4C6::info()};
-extern const int kBF16x4NumTiles = 7;
+extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info(), B4C6::info(), B4C7::info()};
+extern const int kBF16x4NumTiles = 8;
 
 hipError_t run_gemm_bf16x4(const Problem& p) {
   switch (p.cfg) {
@@ -28,6 +30,7 @@ hipError_t run_gemm_bf16x4(const Problem& p) {
     case 4: return B4C4::run(p);
     case 5: return B4C5::run(p);
     case 6: return B4C6::run(p);
+    case 7: return B4C7::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -42,6 +45,7 @@ hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, int total, 
     case 4: return B4C4::grouped(table, count, total, blocks, s);
     case 5: return B4C5::grouped(table, count, total, blocks, s);
     case 6: return B4C6::grouped(table, count, total, blocks, s);
+    case 7: return hipErrorNotSupported;  // pre-split: no grouped launch
     default: return hipErrorInvalidValue;
   }
 }
@@ -68,6 +72,7 @@ int gemm_occupancy_bf16x4(int cfg, int ta, int tb, bool edge) {
     case 4: return B4C4::occupancy(ta, tb, edge);
     case 5: return B4C5::occupancy(ta, tb, edge);
     case 6: return B4C6::occupancy(ta, tb, edge);
+    case 7: return B4C7::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

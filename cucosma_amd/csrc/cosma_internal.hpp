@@ -28,6 +28,9 @@ struct Problem {
   int group_m;
   int act;  // output activation: 0 none, 1 ReLU, 2 sigmoid
   void* ws;
+  void* planes;  // pre-split tile configurations: room for the planes not given below
+  const void* a_planes;  // caller's pre-split op(A) (M rows), or nullptr
+  const void* b_planes;  // caller's pre-split op(B) (N rows), or nullptr
   hipStream_t stream;
 };
 
@@ -38,7 +41,8 @@ struct TileInfo {
   int lds_bytes;
   // Next-stage copy placement: 0 after the MFMAs, 1 after half of them, 2
   // after three quarters, 3 register double-buffered, 4 ping-pong (half the
-  // waves copy before their MFMAs, half after).
+  // waves copy before their MFMAs, half after), 5 pre-split operands copied
+  // with direct-to-LDS loads (no grouped launch).
   int pipe;
 };
 
@@ -57,6 +61,17 @@ struct GroupEntry {
   int pad_;
   double alpha, beta;
 };
+
+// Bytes of bf16 planes a pre-split tile configuration needs: (M + N) rows of
+// 2 * Kp elements, Kp = K rounded up to 32.
+inline size_t planar_bytes(int M, int N, int K) {
+  const int64_t kp = ((int64_t)K + 31) / 32 * 32;
+  return (size_t)((int64_t)M + N) * 2 * (size_t)kp * 2;
+}
+// Split an FP32 operand into planes (kmajor: element (r, k) at X[r*ld + k]).
+hipError_t run_split_operand(int mode, bool kmajor, int rows, int K, const float* X, int64_t ld, void* planes,
+                             hipStream_t s);
+constexpr int kPipePlanar = 5;
 
 int num_tile_cfgs(int mode);
 const TileInfo* tile_info(int mode, int cfg);  // nullptr when out of range

@@ -374,6 +374,42 @@ struct TileShape {
   static_assert(BK % KR == 0, "BK must cover whole fragment rounds");
 };
 
+// The products of one sub-tile row of the split modes for one 32-deep
+// k-step: smallest terms first, then the leading hi*hi term, the product
+// index outermost so TN independent MFMAs separate two updates of one
+// accumulator.  kBF16x3 chains the three products into the running
+// accumulator; kBF16x4 sums its four products of this k-step in a zeroed
+// temporary and adds that with one round-to-nearest VALU add, so the large
+// running sum is rounded once per k-step.
+template <int MODE, int TN>
+__device__ __forceinline__ void split_products(f32x4 (&row)[TN], const bf16x8& ah, const bf16x8& al,
+                                               const bf16x8 (&bh)[TN], const bf16x8 (&bl)[TN]) {
+  if constexpr (MODE == kBF16x4) {
+    f32x4 t[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) t[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j], t[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], t[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], t[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], t[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) row[j][r] += t[j][r];
+  } else {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], row[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], row[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], row[j], 0, 0, 0);
+  }
+}
+
 // MFMA sequence over one LDS stage for one wave's TM x TN sub-tiles.
 //
 // PIPE selects how the copy of the next stage (mid(): split + LDS writes of
@@ -410,39 +446,10 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
   const typename T::S* sa = st;
   const typename T::S* sb = st + BM * BK;
 
-  // One sub-tile row of the split modes: smallest terms first, then the
-  // leading hi*hi term, the product index outermost so TN independent MFMAs
-  // separate two updates of one accumulator.  kBF16x3 chains the three
-  // products into the running accumulator; kBF16x4 sums its four products of
-  // this k-step in a zeroed temporary and adds that with one round-to-nearest
-  // VALU add, so the large running sum is rounded once per k-step.
-  // (Generic lambdas: their bodies are instantiated only for the modes that
-  // call them.)
+  // One sub-tile row of the split modes (split_products).  (Generic lambda:
+  // its body is instantiated only for the modes that call it.)
   auto split_row = [&](auto& row, const auto& ah, const auto& al, const auto& bh, const auto& bl) {
-    if constexpr (MODE == kBF16x4) {
-      Acc t[TN];
-#pragma unroll
-      for (int j = 0; j < TN; ++j) t[j] = zero_acc<MODE>();
-#pragma unroll
-      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j], t[j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], t[j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], t[j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], t[j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) row[j][r] += t[j][r];
-    } else {
-#pragma unroll
-      for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], row[j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], row[j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], row[j], 0, 0, 0);
-    }
+    split_products<MODE, TN>(row, ah, al, bh, bl);
   };
   auto mfma1 = [&](Acc& c, auto a, auto b) {
     if constexpr (MODE == kF32)
@@ -758,6 +765,219 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
     const int tn = local - tm * e.tiles_n;
     gemm_tile<MODE, BM, BN, BK, WM, WN, PIPE, false, false, true>(p, tm, tn, 0, lds);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Pre-split BF16 path.  A pass (split_kmajor / split_mnmajor) writes each FP32
+// operand once as bf16 "planes": row r (an M or N index) of 2*Kp elements,
+// each 32-deep K chunk stored as its 32 hi parts followed by its 32 lo parts
+// (hi = RNE(x), lo = RNE(x - hi), the split the on-the-fly path does), K
+// zero-padded to Kp, a multiple of 32.  Both operands are then K-major, so the
+// GEMM (planar_kernel) copies 16-byte chunks straight from global memory into
+// LDS with global_load_lds_dwordx4: no VALU split, no staging registers, no
+// LDS store instructions.  Products and their order are those of the
+// on-the-fly kernel with the same tile, so the results are bitwise equal.
+// ---------------------------------------------------------------------------
+
+// Operand element (r, k) at X[r * ld + k] (k-contiguous rows).  One thread
+// per (row, 8 consecutive k).
+template <int MODE>
+__global__ void __launch_bounds__(256)
+    split_kmajor(const float* __restrict__ X, int64_t ld, int R, int K, int Kp, int vec,
+                 __bf16* __restrict__ out) {
+  const int cpr = Kp / 8;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)R * cpr) return;
+  const int r = (int)(t / cpr);
+  const int k0 = (int)(t - (int64_t)r * cpr) * 8;
+  const float* x = X + (int64_t)r * ld + k0;
+  f32x4 v0, v1;
+  if (vec && k0 + 8 <= K) {
+    v0 = *reinterpret_cast<const f32x4*>(x);
+    v1 = *reinterpret_cast<const f32x4*>(x + 4);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v0[e] = k0 + e < K ? x[e] : 0.f;
+      v1[e] = k0 + 4 + e < K ? x[4 + e] : 0.f;
+    }
+  }
+  bf16x4 h0, l0, h1, l1;
+  split_bf16x4(v0, h0, l0);
+  split_bf16x4(v1, h1, l1);
+  __bf16* o = out + (int64_t)r * 2 * Kp + (k0 / 32) * 64 + (k0 % 32);
+  *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
+  *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
+}
+
+// Operand element (r, k) at X[k * ld + r] (r-contiguous): transposed through
+// LDS in tiles of 64 rows x 64 k, so reads (16 bytes per lane along r when
+// the rows allow it) and writes (whole 32-k chunks of a row) are contiguous.
+// 1-D grid: nrb = ceil(R / 64) row blocks times ceil(Kp / 64) k blocks.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+    split_mnmajor(const float* __restrict__ X, int64_t ld, int R, int K, int Kp, int nrb, int vec,
+                  __bf16* __restrict__ out) {
+  __shared__ float tile[64][65];
+  const int r0 = (int)(blockIdx.x % (unsigned)nrb) * 64;
+  const int k0 = (int)(blockIdx.x / (unsigned)nrb) * 64;
+  const int tid = threadIdx.x;
+  const bool full = vec && r0 + 64 <= R;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // 64 k-rows x 16 vectors of 4 r
+    const int kk = (tid >> 4) + 16 * i;
+    const int rv = (tid & 15) * 4;
+    const int k = k0 + kk;
+    const float* x = X + (int64_t)k * ld + r0 + rv;
+    f32x4 v;
+    if (full && k < K) {
+      v = *reinterpret_cast<const f32x4*>(x);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = (k < K && r0 + rv + e < R) ? x[e] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) tile[kk][rv + e] = v[e];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {  // 64 rows x 8 chunks of 8 k, two per thread
+    const int rr = tid >> 2;
+    const int c = (tid & 3) + 4 * h;
+    const int r = r0 + rr;
+    const int kc = k0 + c * 8;
+    if (r >= R || kc >= Kp) continue;
+    f32x4 v0, v1;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v0[e] = tile[c * 8 + e][rr];
+      v1[e] = tile[c * 8 + 4 + e][rr];
+    }
+    bf16x4 h0, l0, h1, l1;
+    split_bf16x4(v0, h0, l0);
+    split_bf16x4(v1, h1, l1);
+    __bf16* o = out + (int64_t)r * 2 * Kp + (kc / 32) * 64 + (kc % 32);
+    *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
+    *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
+  }
+}
+
+template <int MODE>
+struct PlanarParams {
+  GemmParams<MODE> g;  // C, ws, M, N, K, tiles, split, kslice, group_m, alpha, beta, act
+  const __bf16* Ap;    // [M][2*Kp]
+  const __bf16* Bp;    // [N][2*Kp]
+  int Kp;
+};
+
+// GEMM over planes.  Workgroup tile BM x BN, waves of WM x WN, two LDS stages
+// of (BM + BN) rows x 64 bf16 (hi and lo of a 32-deep chunk) in the 128-byte
+// row layout of LdsLayout.  Each global_load_lds_dwordx4 of a wave fills one
+// 1-KiB block of 8 LDS rows: lane L writes 16-byte unit L of the block, so it
+// loads the chunk that the layout places there.  Rows beyond M (N) load the
+// last valid row and are never stored.  Loads for chunk it+1 are issued
+// before the MFMAs of chunk it and waited for (vmcnt) before the barrier that
+// ends the iteration.
+template <int MODE, int BM, int BN, int WM, int WN, bool EDGE>
+__device__ __forceinline__ void planar_tile(const PlanarParams<MODE>& pp, __bf16* lds) {
+  constexpr int BK = 32;
+  constexpr int ROWE = 2 * BK;              // elements per LDS row
+  constexpr int STAGE = (BM + BN) * ROWE;
+  constexpr int NT = 64 * (BM / WM) * (BN / WN);
+  constexpr int NW = NT / 64;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int BLK_A = BM / 8, BLK_B = BN / 8;  // 1-KiB blocks per operand and stage
+  static_assert(BLK_A % NW == 0 && BLK_B % NW == 0, "whole blocks per wave");
+  constexpr int PA = BLK_A / NW, PB = BLK_B / NW;
+  using L = LdsLayout<MODE, ROWE>;
+  static_assert(L::NC == 8, "128-byte LDS rows");
+  const GemmParams<MODE>& p = pp.g;
+  int tm, tn, z;
+  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave / (BN / WN)) * WM;
+  const int wn0 = (wave % (BN / WN)) * WN;
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
+  const int64_t rowlen = 2 * (int64_t)pp.Kp;
+
+  // Source of this lane's unit in each of its blocks (chunk 0 of the slice).
+  const int cbeg = z * (p.kslice / BK);
+  const int cend = min(pp.Kp / BK, cbeg + p.kslice / BK);
+  const int nk = cend > cbeg ? cend - cbeg : 0;
+  const int rin = lane >> 3;
+  const __bf16* srcA[PA];
+  const __bf16* srcB[PB];
+#pragma unroll
+  for (int j = 0; j < PA; ++j) {
+    const int r = 8 * (wave * PA + j) + rin;
+    const int c = (lane & 7) ^ (((r >> 1) & 3) << 1);  // inverse of LdsLayout (an involution)
+    srcA[j] = pp.Ap + (int64_t)min(m0 + r, p.M - 1) * rowlen + (int64_t)cbeg * ROWE + c * 8;
+  }
+#pragma unroll
+  for (int j = 0; j < PB; ++j) {
+    const int r = 8 * (wave * PB + j) + rin;
+    const int c = (lane & 7) ^ (((r >> 1) & 3) << 1);
+    srcB[j] = pp.Bp + (int64_t)min(n0 + r, p.N - 1) * rowlen + (int64_t)cbeg * ROWE + c * 8;
+  }
+  auto issue = [&](int it, int s) {
+    __bf16* st = lds + s * STAGE;
+#pragma unroll
+    for (int j = 0; j < PA; ++j)
+      __builtin_amdgcn_global_load_lds(srcA[j] + it * ROWE, st + (wave * PA + j) * 8 * ROWE, 16, 0, 0);
+#pragma unroll
+    for (int j = 0; j < PB; ++j)
+      __builtin_amdgcn_global_load_lds(srcB[j] + it * ROWE, st + BM * ROWE + (wave * PB + j) * 8 * ROWE, 16,
+                                       0, 0);
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int g = lane >> 4;
+  const int r16 = lane & 15;
+  if (nk > 0) {
+    issue(0, 0);
+    // The direct loads write LDS behind the compiler's back: wait for them
+    // and fence compiler reordering before the barrier publishes the stage.
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+  for (int it = 0; it < nk; ++it) {
+    const int cur = it & 1;
+    if (it + 1 < nk) issue(it + 1, cur ^ 1);
+    const __bf16* sa = lds + cur * STAGE;
+    const __bf16* sb = sa + BM * ROWE;
+    bf16x8 bh[TN], bl[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int row = wn0 + j * 16 + r16;
+      bh[j] = *reinterpret_cast<const bf16x8*>(sb + L::off(row, g * 8));
+      bl[j] = *reinterpret_cast<const bf16x8*>(sb + L::off(row, BK + g * 8));
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int row = wm0 + i * 16 + r16;
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(sa + L::off(row, g * 8));
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(sa + L::off(row, BK + g * 8));
+      split_products<MODE, TN>(acc[i], ah, al, bh, bl);
+    }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+  epilogue<MODE, EDGE, TM, TN>(p, z, m0 + wm0, n0 + wn0, lane, acc);
+}
+
+// The pre-split GEMM kernel: grid tiles_m * tiles_n * split (1-D), two LDS
+// stages of (BM + BN) x 64 bf16.
+template <int MODE, int BM, int BN, int WM, int WN, bool EDGE>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) planar_kernel(PlanarParams<MODE> pp) {
+  __shared__ __attribute__((aligned(16))) __bf16 lds[2 * (BM + BN) * 64];
+  planar_tile<MODE, BM, BN, WM, WN, EDGE>(pp, lds);
 }
 
 }  // namespace cosma

@@ -104,4 +104,101 @@ struct Tile {
   }
 };
 
+// Pre-split BF16 tile configuration: split pass into pr.planes, then
+// planar_kernel (and the reduction when split > 1).
+template <int MODE, int BM, int BN, int WM, int WN>
+struct Planar {
+  static constexpr int kThreads = 64 * (BM / WM) * (BN / WN);
+  static constexpr TileInfo info() {
+    return TileInfo{BM, BN, 32, WM, WN, kThreads, (int)(2 * (BM + BN) * 64 * sizeof(__bf16)), kPipePlanar};
+  }
+  template <bool EDGE>
+  static int occ() {
+    int n = 0;
+    void (*fn)(PlanarParams<MODE>) = planar_kernel<MODE, BM, BN, WM, WN, EDGE>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(fn), kThreads, 0) != hipSuccess)
+      return 0;
+    return n;
+  }
+  static int occupancy(int, int, bool edge) { return edge ? occ<true>() : occ<false>(); }
+
+  // Split operand X into planes: kmajor when element (r, k) is X[r*ld + k].
+  static hipError_t split(const float* X, int64_t ld, bool kmajor, int R, int K, int Kp, __bf16* out,
+                          hipStream_t s) {
+    if (kmajor) {
+      const int64_t threads = (int64_t)R * (Kp / 8);
+      const int64_t blocks = (threads + 255) / 256;
+      if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+      const int vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
+      hipLaunchKernelGGL(split_kmajor<MODE>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, R, K, Kp, vec, out);
+    } else {
+      const int nrb = (R + 63) / 64;
+      const int64_t blocks = (int64_t)nrb * ((Kp + 63) / 64);
+      if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+      const int vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
+      hipLaunchKernelGGL(split_mnmajor<MODE>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, R, K, Kp, nrb, vec,
+                         out);
+    }
+    return hipGetLastError();
+  }
+
+  static hipError_t run(const Problem& pr) {
+    const int Kp = (int)(((int64_t)pr.K + 31) / 32 * 32);
+    __bf16* room = static_cast<__bf16*>(pr.planes);
+    const __bf16* Ap = static_cast<const __bf16*>(pr.a_planes);
+    const __bf16* Bp = static_cast<const __bf16*>(pr.b_planes);
+    hipError_t e = hipSuccess;
+    if (!Ap) {
+      e = split(static_cast<const float*>(pr.A), pr.lda, pr.ta == 0, pr.M, pr.K, Kp, room, pr.stream);
+      if (e != hipSuccess) return e;
+      Ap = room;
+      room += (int64_t)pr.M * 2 * Kp;
+    }
+    if (!Bp) {
+      e = split(static_cast<const float*>(pr.B), pr.ldb, pr.tb == 1, pr.N, pr.K, Kp, room, pr.stream);
+      if (e != hipSuccess) return e;
+      Bp = room;
+    }
+    PlanarParams<MODE> pp;
+    GemmParams<MODE>& gp = pp.g;
+    gp.A = static_cast<const float*>(pr.A);
+    gp.B = static_cast<const float*>(pr.B);
+    gp.C = static_cast<float*>(pr.C);
+    gp.ws = static_cast<float*>(pr.ws);
+    gp.lda = pr.lda;
+    gp.ldb = pr.ldb;
+    gp.ldc = pr.ldc;
+    gp.M = pr.M;
+    gp.N = pr.N;
+    gp.K = pr.K;
+    gp.tiles_m = (pr.M + BM - 1) / BM;
+    gp.tiles_n = (pr.N + BN - 1) / BN;
+    gp.split = pr.split;
+    const int per = (pr.K + pr.split - 1) / pr.split;
+    gp.kslice = ((per + 31) / 32) * 32;
+    gp.group_m = pr.group_m > 0 ? pr.group_m : 8;
+    gp.vec_a = 1;
+    gp.vec_b = 1;
+    gp.act = pr.act;
+    gp.alpha = (float)pr.alpha;
+    gp.beta = (float)pr.beta;
+    pp.Ap = Ap;
+    pp.Bp = Bp;
+    pp.Kp = Kp;
+    const int64_t nblk64 = (int64_t)gp.tiles_m * gp.tiles_n * pr.split;
+    if (nblk64 <= 0 || nblk64 > 0x7fffffff) return hipErrorInvalidValue;
+    const bool edge = (pr.M % BM) || (pr.N % BN);
+    if (edge)
+      hipLaunchKernelGGL((planar_kernel<MODE, BM, BN, WM, WN, true>), dim3((unsigned)nblk64), dim3(kThreads), 0,
+                         pr.stream, pp);
+    else
+      hipLaunchKernelGGL((planar_kernel<MODE, BM, BN, WM, WN, false>), dim3((unsigned)nblk64), dim3(kThreads), 0,
+                         pr.stream, pp);
+    e = hipGetLastError();
+    if (e != hipSuccess || pr.split == 1) return e;
+    return run_reduce_f32(static_cast<const float*>(pr.ws), static_cast<float*>(pr.C), pr.ldc, pr.M, pr.N,
+                          pr.split, (float)pr.alpha, (float)pr.beta, pr.act, pr.stream);
+  }
+};
+
 }  // namespace cosma

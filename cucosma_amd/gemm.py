@@ -47,6 +47,8 @@ def gemm(
     workspace: torch.Tensor | None = None,
     stream=None,
     epilogue: str | None = None,
+    a_planes: torch.Tensor | None = None,
+    b_planes: torch.Tensor | None = None,
 ) -> torch.Tensor:
     """C = act(alpha * op(A) @ op(B) + beta * C) on the GPU.
 
@@ -58,6 +60,8 @@ def gemm(
     split_k <= 0 and tile_cfg < 0 select the library heuristic.
     epilogue: None, "relu" (max(x, 0)) or "sigmoid" (1 / (1 + exp(-x))), fused
     into the kernel's store (after the split-K reduction when K is split).
+    a_planes / b_planes: split-mode operands already split by split_operand
+    (op(A) and op(B) respectively); A and B still describe the problem.
     """
     if mode not in MODES:
         raise ValueError(f"unknown mode {mode!r}")
@@ -90,6 +94,11 @@ def gemm(
     d.alpha, d.beta = float(alpha), float(beta)
     d.split_k, d.tile_cfg, d.group_m = int(split_k), int(tile_cfg), int(group_m)
     d.epilogue = EPILOGUES[epilogue]
+    for t, name in ((a_planes, "a_planes"), (b_planes, "b_planes")):
+        if t is not None and not (t.is_cuda and t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous device tensor from split_operand")
+    d.a_planes = a_planes.data_ptr() if a_planes is not None else None
+    d.b_planes = b_planes.data_ptr() if b_planes is not None else None
     if workspace is not None:
         d.workspace = workspace.data_ptr()
         d.workspace_bytes = workspace.numel() * workspace.element_size()
@@ -182,3 +191,30 @@ def fp64_product(A: torch.Tensor, B: torch.Tensor, C0: torch.Tensor | None = Non
     R = to_f64(C0, stream) if (C0 is not None and beta != 0.0) else None
     return gemm(A64, B64, R, alpha=alpha, beta=beta if R is not None else 0.0, mode="fp64",
                 stream=stream)
+
+
+def split_operand(X: torch.Tensor, *, trans: bool = False, as_b: bool = False, mode: str = "bf16x3",
+                  stream=None) -> torch.Tensor:
+    """Split an FP32 operand once into the bf16 hi/lo planes of the pre-split
+    path; pass the result as gemm(..., a_planes=) or b_planes=.
+
+    As A (as_b False): X is A (M x K), or with trans its stored K x M form.
+    As B (as_b True): X is B (K x N), or with trans its stored N x K form.
+    Returns a bfloat16 tensor of rows x 2*Kp (rows = M or N, Kp = K rounded
+    up to 32).
+    """
+    if mode not in ("bf16x3", "bf16x4"):
+        raise ValueError("split_operand is for the split modes")
+    if X.dtype != torch.float32 or not X.is_cuda:
+        raise TypeError("X must be a float32 device tensor")
+    if as_b:
+        rows, k = (X.shape[0], X.shape[1]) if trans else (X.shape[1], X.shape[0])
+        kmajor = trans
+    else:
+        rows, k = (X.shape[1], X.shape[0]) if trans else (X.shape[0], X.shape[1])
+        kmajor = not trans
+    kp = (k + 31) // 32 * 32
+    out = torch.empty((rows, 2 * kp), dtype=torch.bfloat16, device=X.device)
+    check(lib.cosma_split_operand(MODES[mode], int(kmajor), rows, k, X.data_ptr(), _ld(X, "X"),
+                                  out.data_ptr(), _stream_ptr(stream)), "cosma_split_operand")
+    return out

@@ -87,6 +87,11 @@ typedef struct cosma_gemm_desc {
     size_t workspace_bytes; /* size of workspace */
     void *stream;           /* hipStream_t; NULL: default stream */
     int epilogue;           /* COSMA_EPILOGUE_*: activation applied to the final C */
+    /* BF16X3 / BF16X4 only: operands already split by cosma_split_operand
+     * (NULL: split inside the call when the tile configuration works on
+     * pre-split operands).  Setting either selects such a configuration. */
+    const void *a_planes;   /* op(A) as M rows of K, split */
+    const void *b_planes;   /* op(B) as N rows of K (i.e. B transposed), split */
 } cosma_gemm_desc;
 
 /* Fill *d with defaults (heuristic split and tile, default stream). */
@@ -106,6 +111,21 @@ int cosma_workspace_query(const cosma_gemm_desc *d, size_t *bytes);
  * while the stream is capturing.  Calls that share a workspace must be
  * ordered on one stream. */
 int cosma_workspace_set(void *ptr, size_t bytes);
+
+/* Pre-split operands (BF16X3 / BF16X4).  The split modes can split each FP32
+ * operand once into bf16 hi/lo "planes" -- rows of 2 * Kp bf16, Kp = K
+ * rounded up to 32, each 32-deep K chunk stored as 32 hi then 32 lo parts --
+ * and run the GEMM on planes copied straight into LDS.  Large GEMMs do this
+ * inside the call (the tile configuration with pipeline schedule 5); a
+ * caller that multiplies one operand several times can split it once with
+ * cosma_split_operand and pass the planes in the descriptor.
+ *   element (r, k) of the operand is X[r * ld + k] when kmajor != 0, else
+ *   X[k * ld + r]; planes must hold cosma_planes_bytes(rows, k) bytes,
+ *   16-byte aligned.  For A pass rows = M (kmajor = !trans_a); for B pass
+ *   rows = N (kmajor = trans_b).  Asynchronous on stream. */
+size_t cosma_planes_bytes(int rows, int k);
+int cosma_split_operand(int mode, int kmajor, int rows, int k, const float *X, int64_t ld, void *planes,
+                        void *stream);
 
 /* Fixed-shape entry points: the shape is fixed when the library is built
  * (COSMA_M, COSMA_N, COSMA_K; default 1024 x 1024 x 1024), as the reference

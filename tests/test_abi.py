@@ -102,6 +102,9 @@ def _desc(**kw):
         (dict(tile_cfg=99), -1),
         (dict(epilogue=3), -1),
         (dict(epilogue=-1), -1),
+        (dict(a_planes=0x5000), -1),  # planes with a non-split mode
+        (dict(mode=2, b_planes=0x5008), -2),
+        (dict(mode=2, a_planes=0x5000, tile_cfg=6), -1),  # planes need the pre-split tile
         (dict(a=0x1002), -2),
         (dict(b=0x2001), -2),
         (dict(mode=1, a=0x1004), -2),

@@ -225,6 +225,89 @@ def test_fused_epilogue(cc, dev, mode, dt, act):
         cc.gemm(Ad, Bd, Cd, mode=mode, epilogue="tanh")
 
 
+PRESPLIT = {"bf16x3": (7, 6), "bf16x4": (7, 3)}  # pre-split cfg, on-the-fly cfg with the same tile
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
+@pytest.mark.parametrize("ta,tb", LAYOUTS)
+def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
+    """The pre-split configuration (split pass into bf16 planes, direct-to-LDS
+    loads) computes the same products in the same order as the on-the-fly
+    kernel with the same 256x256 tile: bitwise equal results, for aligned and
+    ragged shapes, with split-K, alpha and beta."""
+    planar, same = PRESPLIT[mode]
+    t = cc.tile_configs(cc._lib.MODES[mode])
+    assert t[planar]["pipe"] == 5 and (t[planar]["BM"], t[planar]["BN"]) == (t[same]["BM"], t[same]["BN"])
+    rng = np.random.default_rng(400)
+    for (M, N, K, split) in ((512, 512, 256, 1), (300, 200, 100, 1), (256, 512, 1000, 3), (33, 517, 77, 2)):
+        A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+        B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        C0 = rng.uniform(-1, 1, (M, N)).astype(np.float32)
+        Ad, Bd = _t(A.T if ta else A, dev), _t(B.T if tb else B, dev)
+        outs = []
+        for cfg in (planar, same):
+            Cd = _t(C0, dev)
+            cc.gemm(Ad, Bd, Cd, alpha=1.25, beta=0.5, mode=mode, trans_a=ta, trans_b=tb, tile_cfg=cfg,
+                    split_k=split)
+            outs.append(Cd.cpu().numpy())
+        assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (M, N, K, split)
+        ref = oracle.gemm_bf16x3(A, B, C0, 1.25, 0.5) if mode == "bf16x3" else oracle.gemm_bf16x4(A, B, C0, 1.25, 0.5)
+        absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64)) + 0.5 * np.abs(C0)
+        assert np.all(np.abs(outs[0] - ref) <= 2 * (K + 2) * 2.0**-24 * 1.25 * absprod + 1e-30)
+
+
+@pytest.mark.parametrize("ta,tb", LAYOUTS)
+def test_presplit_caller_planes(cc, dev, ta, tb):
+    """Operands split once by split_operand and passed as planes give the
+    bitwise result of the in-call split; one or both operands may be given."""
+    rng = np.random.default_rng(401)
+    M, N, K = 300, 520, 200
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    Ad, Bd = _t(A.T if ta else A, dev), _t(B.T if tb else B, dev)
+    ap = cc.split_operand(Ad, trans=ta)
+    bp = cc.split_operand(Bd, trans=tb, as_b=True)
+    kp = (K + 31) // 32 * 32
+    assert tuple(ap.shape) == (M, 2 * kp) and tuple(bp.shape) == (N, 2 * kp)
+    assert ap.numel() * 2 == cc.lib.cosma_planes_bytes(M, K)
+    want = cc.gemm(Ad, Bd, mode="bf16x3", trans_a=ta, trans_b=tb, tile_cfg=7).cpu().numpy()
+    for kw in (dict(a_planes=ap, b_planes=bp), dict(a_planes=ap), dict(b_planes=bp)):
+        got = cc.gemm(Ad, Bd, mode="bf16x3", trans_a=ta, trans_b=tb, **kw).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), list(kw)
+    # the planes hold the RNE hi/lo split of the operand, K zero-padded
+    hi, lo = oracle.split_bf16(A)
+    planes = ap.view(torch.int16).cpu().numpy().view(np.uint16).reshape(M, kp // 32, 64)
+    hp, lp = planes[:, :, :32].reshape(M, kp)[:, :K], planes[:, :, 32:].reshape(M, kp)[:, :K]
+    assert np.array_equal(hp, hi) and np.array_equal(lp, lo)
+    assert not planes[:, :, :32].reshape(M, kp)[:, K:].any()
+
+
+def test_presplit_workspace_and_grouped_rejection(cc, dev):
+    from cucosma_amd._lib import GemmDesc
+
+    M, N, K = 1000, 600, 333
+    d = GemmDesc()
+    cc.lib.cosma_gemm_desc_init(C.byref(d))
+    d.mode, d.m, d.n, d.k = cc._lib.MODE_BF16X3, M, N, K
+    d.a, d.b, d.c = 0x1000, 0x2000, 0x3000
+    d.lda, d.ldb, d.ldc = K, N, N
+    d.tile_cfg, d.split_k = 7, 1
+    size = C.c_size_t()
+    assert cc.lib.cosma_workspace_query(C.byref(d), C.byref(size)) == 0
+    kp = (K + 31) // 32 * 32
+    assert size.value == (M + N) * 2 * kp * 2
+    d.split_k = 4
+    assert cc.lib.cosma_workspace_query(C.byref(d), C.byref(size)) == 0
+    part = 4 * M * N * 4
+    assert size.value == (part + 255) // 256 * 256 + (M + N) * 2 * kp * 2
+    A = torch.ones((64, 32), device=dev)
+    B = torch.ones((32, 48), device=dev)
+    Cm = torch.zeros((64, 48), device=dev)
+    with pytest.raises(cc.CosmaError) as e:
+        cc.GroupedGemm([A], [B], [Cm], mode="bf16x3", tile_cfgs=[7])
+    assert e.value.status == -3
+
+
 def test_residual_stats_on_gpu_match_host(cc, dev):
     rng = np.random.default_rng(103)
     R = rng.normal(0, 1, (513, 257))
