@@ -41,7 +41,7 @@ constexpr int kCUs = 256;  // MI355X compute units
 struct TileDims {
   int bm, bn, bk;
 };
-constexpr int kMaxCfg = 8;
+constexpr int kMaxCfg = 9;
 constexpr TileDims kDims[4][kMaxCfg] = {
     // FP32
     {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}, {256, 128, 32},
@@ -51,13 +51,13 @@ constexpr TileDims kDims[4][kMaxCfg] = {
      {256, 128, 16}, {128, 128, 16}},
     // BF16X3 (cfg 7: pre-split operands)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}, {256, 256, 32}},
+     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}},
     // BF16X4 (same tiles as BF16X3)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
      {256, 256, 32}, {256, 256, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {7, 8, 8, 8};
+constexpr int kNumCfg[kNumModes] = {7, 8, 9, 8};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -78,10 +78,11 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    is read from HBM once).  It beat every smaller tile from 2048^3 up, and
 //    for 4096x4096x16384 (the weight gradient of a 16384x4096x4096 layer) ran
 //    at twice the speed of 128x128.  When M*N/(M+N) >= 3072 (8192^3,
-//    16384x4096, 12288^3, ...) the same tile runs over pre-split operands
-//    (cfg 7): the split pass costs (M+N)*K*8 bytes of HBM traffic, and the
-//    GEMM without the in-loop split is 16% faster (8192^3: 2.48 vs 2.64 ms in
-//    total; 4096^3 and 2048x8192x2048 lose 7-15%, so they keep cfg 6);
+//    16384x4096, 12288^3, ...) the 256x256 tile runs over pre-split operands
+//    (planar_cfg): the split pass costs (M+N)*K*8 bytes of HBM traffic, and
+//    the GEMM without the in-loop split is 16% faster with 8 waves, 22% with
+//    16 (8192^3: 2.35 ms in total against 2.64 for cfg 6; 4096^3 and
+//    2048x8192x2048 lose to cfg 6, which they keep);
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    (8 waves of 64x64, next stage held in registers; at 8192^3 it matched
 //    the copy-between-MFMAs variant with A row-major and beat it by 12% with
@@ -102,9 +103,11 @@ struct Choice {
   int cfg, split;
 };
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
-// The tile configuration over pre-split operands (no grouped launch).
-constexpr int kPlanarCfg = 7;
-constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && cfg == kPlanarCfg; }
+// Tile configurations over pre-split operands (cfg >= 7; no grouped launch)
+// and the one the heuristic uses: BF16x3 16 waves of 64x64 (117 VGPRs, four
+// waves per SIMD), BF16x4 8 waves of 128x64 (its 16-wave version spills).
+constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && cfg >= 7; }
+constexpr int planar_cfg(int mode) { return mode == 2 ? 8 : 7; }
 constexpr int large_cfg(int mode) { return mode == 0 ? 5 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
@@ -128,7 +131,8 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     const int split = lt >= kCUs / 2 ? 1 : clamp_split(kCUs / lt, k, 512, kCUs);
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2) {
       // Large outputs: split the operands once and copy planes into LDS.
-      if (split == 1 && lt >= kCUs && (int64_t)m * n >= 3072 * ((int64_t)m + n)) return Choice{kPlanarCfg, 1};
+      if (split == 1 && lt >= kCUs && (int64_t)m * n >= 3072 * ((int64_t)m + n))
+        return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }
   } else if (lt >= 2 * kCUs) {
@@ -292,7 +296,7 @@ void resolve_choice(const cosma_gemm_desc* d, int* cfg, int* split, int* group_m
     if (*group_m <= 0) *group_m = tv.group_m;
   }
   *cfg = d->tile_cfg >= 0 ? d->tile_cfg : ch.cfg;
-  if ((d->a_planes || d->b_planes) && !is_planar(d->mode, *cfg)) *cfg = kPlanarCfg;
+  if ((d->a_planes || d->b_planes) && !is_planar(d->mode, *cfg)) *cfg = planar_cfg(d->mode);
   *split = d->split_k > 0 ? d->split_k : (d->tile_cfg >= 0 ? 1 : ch.split);
   if (*split > d->k) *split = d->k > 0 ? d->k : 1;
 }

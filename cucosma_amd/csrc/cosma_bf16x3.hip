@@ -10,6 +10,8 @@
 // cfg 6: cfg 3 with ping-pong copies (PIPE 4): the large-shape tile
 // cfg 7: 256x256, 8 waves of 128x64 over pre-split operands (split pass, then
 //        direct-to-LDS loads; Planar in cosma_launch.hpp); not for grouped plans
+// cfg 8: cfg 7 with 16 waves of 64x64 (four waves per SIMD): the large-output
+//        configuration (8192^3: planar kernel 508 TFLOP/s against 479 for cfg 7)
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -22,9 +24,10 @@ using B3C4 = Tile<kBF16x3, 256, 128, 32, 128, 64>;
 using B3C5 = Tile<kBF16x3, 128, 128, 64, 64, 64>;
 using B3C6 = Tile<kBF16x3, 256, 256, 32, 128, 64, 4>;
 using B3C7 = Planar<kBF16x3, 256, 256, 128, 64>;
+using B3C8 = Planar<kBF16x3, 256, 256, 64, 64>;
 
-extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info(), B3C6::info(), B3C7::info()};
-extern const int kBF16x3NumTiles = 8;
+extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info(), B3C6::info(), B3C7::info(), B3C8::info()};
+extern const int kBF16x3NumTiles = 9;
 
 hipError_t run_gemm_bf16x3(const Problem& p) {
   switch (p.cfg) {
@@ -36,6 +39,7 @@ hipError_t run_gemm_bf16x3(const Problem& p) {
     case 5: return B3C5::run(p);
     case 6: return B3C6::run(p);
     case 7: return B3C7::run(p);
+    case 8: return B3C8::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -51,6 +55,7 @@ hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, 
     case 5: return B3C5::grouped(table, count, total, blocks, s);
     case 6: return B3C6::grouped(table, count, total, blocks, s);
     case 7: return hipErrorNotSupported;  // pre-split: no grouped launch
+    case 8: return hipErrorNotSupported;
     default: return hipErrorInvalidValue;
   }
 }
@@ -78,6 +83,7 @@ int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge) {
     case 5: return B3C5::occupancy(ta, tb, edge);
     case 6: return B3C6::occupancy(ta, tb, edge);
     case 7: return B3C7::occupancy(ta, tb, edge);
+    case 8: return B3C8::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

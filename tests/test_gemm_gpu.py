@@ -225,7 +225,8 @@ def test_fused_epilogue(cc, dev, mode, dt, act):
         cc.gemm(Ad, Bd, Cd, mode=mode, epilogue="tanh")
 
 
-PRESPLIT = {"bf16x3": (7, 6), "bf16x4": (7, 3)}  # pre-split cfg, on-the-fly cfg with the same tile
+# pre-split configurations and the on-the-fly configuration with the same 256x256 tile
+PRESPLIT = {"bf16x3": ((7, 8), 6), "bf16x4": ((7,), 3)}
 
 
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
@@ -235,9 +236,11 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
     loads) computes the same products in the same order as the on-the-fly
     kernel with the same 256x256 tile: bitwise equal results, for aligned and
     ragged shapes, with split-K, alpha and beta."""
-    planar, same = PRESPLIT[mode]
+    planars, same = PRESPLIT[mode]
     t = cc.tile_configs(cc._lib.MODES[mode])
-    assert t[planar]["pipe"] == 5 and (t[planar]["BM"], t[planar]["BN"]) == (t[same]["BM"], t[same]["BN"])
+    assert [c["cfg"] for c in t if c["pipe"] == 5] == list(planars)
+    for planar in planars:
+        assert (t[planar]["BM"], t[planar]["BN"]) == (t[same]["BM"], t[same]["BN"])
     rng = np.random.default_rng(400)
     for (M, N, K, split) in ((512, 512, 256, 1), (300, 200, 100, 1), (256, 512, 1000, 3), (33, 517, 77, 2)):
         A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
@@ -245,12 +248,13 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
         C0 = rng.uniform(-1, 1, (M, N)).astype(np.float32)
         Ad, Bd = _t(A.T if ta else A, dev), _t(B.T if tb else B, dev)
         outs = []
-        for cfg in (planar, same):
+        for cfg in (same,) + planars:
             Cd = _t(C0, dev)
             cc.gemm(Ad, Bd, Cd, alpha=1.25, beta=0.5, mode=mode, trans_a=ta, trans_b=tb, tile_cfg=cfg,
                     split_k=split)
             outs.append(Cd.cpu().numpy())
-        assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (M, N, K, split)
+        for o in outs[1:]:
+            assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32)), (M, N, K, split)
         ref = oracle.gemm_bf16x3(A, B, C0, 1.25, 0.5) if mode == "bf16x3" else oracle.gemm_bf16x4(A, B, C0, 1.25, 0.5)
         absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64)) + 0.5 * np.abs(C0)
         assert np.all(np.abs(outs[0] - ref) <= 2 * (K + 2) * 2.0**-24 * 1.25 * absprod + 1e-30)
