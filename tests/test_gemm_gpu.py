@@ -255,6 +255,14 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
             outs.append(Cd.cpu().numpy())
         for o in outs[1:]:
             assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32)), (M, N, K, split)
+        # fused epilogue on both paths
+        acts = []
+        for cfg in (same, planars[-1]):
+            Cd = _t(C0, dev)
+            cc.gemm(Ad, Bd, Cd, alpha=1.25, beta=0.5, mode=mode, trans_a=ta, trans_b=tb, tile_cfg=cfg,
+                    split_k=split, epilogue="sigmoid")
+            acts.append(Cd.cpu().numpy())
+        assert np.array_equal(acts[0].view(np.uint32), acts[1].view(np.uint32)), (M, N, K, split)
         ref = oracle.gemm_bf16x3(A, B, C0, 1.25, 0.5) if mode == "bf16x3" else oracle.gemm_bf16x4(A, B, C0, 1.25, 0.5)
         absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64)) + 0.5 * np.abs(C0)
         assert np.all(np.abs(outs[0] - ref) <= 2 * (K + 2) * 2.0**-24 * 1.25 * absprod + 1e-30)
