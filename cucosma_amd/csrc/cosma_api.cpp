@@ -45,7 +45,7 @@ constexpr int kMaxCfg = 9;
 constexpr TileDims kDims[4][kMaxCfg] = {
     // FP32
     {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}, {256, 128, 32},
-     {256, 128, 32}},
+     {256, 128, 32}, {256, 128, 32}},
     // FP64
     {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 16}, {64, 64, 32}, {128, 128, 32},
      {256, 128, 16}, {128, 128, 16}},
@@ -57,7 +57,7 @@ constexpr TileDims kDims[4][kMaxCfg] = {
      {256, 256, 32}, {256, 256, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {7, 8, 9, 8};
+constexpr int kNumCfg[kNumModes] = {8, 8, 9, 8};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -84,10 +84,11 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    16 (8192^3: 2.35 ms in total against 2.64 for cfg 6; 4096^3 and
 //    2048x8192x2048 lose to cfg 6, which they keep);
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
-//    (8 waves of 64x64, next stage held in registers; at 8192^3 it matched
-//    the copy-between-MFMAs variant with A row-major and beat it by 12% with
-//    B stored transposed), FP64 128x128 (8 waves of 64x32, 4 waves per SIMD,
-//    ping-pong copies).  FP32 splits K in two when
+//    in 16 waves of 64x32 with the next stage held in registers (four waves
+//    per SIMD; 3-5% above the 8-wave version, which itself matched the
+//    copy-between-MFMAs variant with A row-major and beat it by 12% with B
+//    stored transposed), FP64 128x128 (8 waves of 64x32, 4 waves per SIMD,
+//    ping-pong copies; a 16-wave version of it ran 12% slower).  FP32 splits K in two when
 //    the last round of workgroups would be at most half full (6144^3: 4.5
 //    rounds of 256 become 9; +7%);
 //  - deep-K outputs (few tiles, K >= 16384): FP32 splits K over 256x128 with
@@ -108,7 +109,7 @@ constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
 // waves per SIMD), BF16x4 8 waves of 128x64 (its 16-wave version spills).
 constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && cfg >= 7; }
 constexpr int planar_cfg(int mode) { return mode == 2 ? 8 : 7; }
-constexpr int large_cfg(int mode) { return mode == 0 ? 5 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
+constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
 // with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
