@@ -9,8 +9,10 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <mutex>
+#include <queue>
 #include <tuple>
 #include <vector>
 
@@ -97,9 +99,11 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    (128x64x32, or 128x128x64 for BF16) until every CU holds two (128x64) or
 //    one (128x128x64) workgroups;
 //  - otherwise 128x128 (BF16) / 128x64 tiles while they give every CU a
-//    workgroup, else 64x64; split K when the 64x64 grid still leaves CUs idle
-//    (FP32/FP64) or gives them fewer than two workgroups (BF16), keeping every
-//    K slice at least 512 deep.
+//    workgroup, else 64x64; split K when the tiles give the CUs fewer than
+//    two workgroups each, aiming at three, with K slices at least 256 deep
+//    (1024^3 with 64x64 tiles: FP32 split 3 ran in 0.038 ms against 0.050
+//    unsplit, BF16x3 split 3 in 0.031 against 0.035 with split 2; cache
+//    flushed, beta 0.5).
 struct Choice {
   int cfg, split;
 };
@@ -150,9 +154,8 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
   cfg = medium_cfg(mode);
   if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;  // 64x64
   const int tiles = tiles_of(mode, cfg, m, n);
-  const int want = split_mode(mode) ? 2 * kCUs : kCUs;
   int split = 1;
-  if (tiles < want && k >= 1024) split = clamp_split(cdiv(4 * kCUs, tiles), k, 512);
+  if (tiles < 2 * kCUs && k >= 1024) split = clamp_split(cdiv(3 * kCUs, tiles), k, 256);
   return Choice{cfg, split};
 }
 
@@ -420,14 +423,32 @@ int finish_stats(hipError_t e, const double* dev_out, double* out5, hipStream_t 
   return (int)hipStreamSynchronize(s);
 }
 
-int group_cfg(int mode, int m, int n) {
-  // BF16 modes: outputs with a side of at least 256 take the 256x256 tile
-  // (on 512 problems with sides in [32, 2048] that bucket ran 1.9x faster
-  // than 128x128 tiles).  Otherwise outputs spanning at least 4 tiles of
-  // 128x128 take that tile, smaller ones 64x64, which keeps more workgroups
-  // busy.
-  if (split_mode(mode) && (m >= 256 || n >= 256)) return large_cfg(mode);
-  return (cdiv(m, 128) * cdiv(n, 128) >= 4) ? 0 : 1;
+// Per-tile cost beyond its K-steps in the grouped schedule (prologue load
+// and C store), in K-steps.
+constexpr int kTileOverheadSteps = 2;
+
+// Default tile of a grouped plan: one configuration for the whole batch, so
+// the plan is a single persistent launch balanced by one schedule (separate
+// buckets run as concurrent launches that compete for CUs without
+// coordination).  The large tile when outputs with a side of at least 256
+// hold half the batch's flops, else 128x128 when outputs with a side of at
+// least 128 do, else 64x64.  On 512 problems with sides log-uniform in
+// [32, 2048] the one large-tile launch ran at 76 / 44 / 118 / 110 TFLOP/s
+// (FP32 / FP64 / BF16x3 / BF16x4) against 59 / 22 / 94 / 88 for per-size
+// buckets of 64x64, 128x128 and large tiles, and within 2% of the best of
+// every combination of those tiles over three size classes.
+int group_default_cfg(int mode, int count, const cosma_group_problem* problems) {
+  double all = 0.0, from256 = 0.0, from128 = 0.0;
+  for (int i = 0; i < count; ++i) {
+    const cosma_group_problem& q = problems[i];
+    const double f = (double)q.m * q.n * (double)q.k;
+    const int side = std::max(q.m, q.n);
+    all += f;
+    if (side >= 256) from256 += f;
+    if (side >= 128) from128 += f;
+  }
+  if (2.0 * from256 >= all) return large_cfg(mode);
+  return 2.0 * from128 >= all ? 0 : 1;
 }
 
 }  // namespace
@@ -439,7 +460,8 @@ struct cosma_grouped_plan_s {
     int count;
     int tiles;
     int blocks;
-    void* table;  // device GroupEntry array
+    void* table;        // device GroupEntry array, then the schedule
+    const int* sched;   // blocks + 1 offsets, then tile ids (inside table's allocation)
   };
   std::vector<Bucket> buckets;
   double flops = 0.0;
@@ -449,10 +471,17 @@ struct cosma_grouped_plan_s {
 
 namespace {
 
-// Tiles are numbered problem by problem and handed to the persistent
-// workgroups round-robin, so the problems are ordered by decreasing K (the
-// cost of each of their tiles): the long tiles start first and the short ones
-// fill the tail (longest-processing-time-first).
+// Tiles are numbered problem by problem, problems ordered by decreasing K
+// (the cost of each of their tiles).  The schedule hands them out
+// longest-processing-time first: each tile, in numbering order, goes to the
+// workgroup with the least work so far (K-steps plus a fixed per-tile cost
+// for the prologue and the C store), ties to the lowest logical workgroup.
+// Logical workgroup L runs as workgroup (L % (B/8)) * 8 + L / (B/8) when B,
+// the grid, is a multiple of 8: workgroups are dealt to the 8 XCDs
+// round-robin by id, so logical neighbours -- which receive the neighbouring
+// tiles of one problem in each round -- share one XCD's L2.  On 512 problems
+// with sides in [32, 2048] (256x256 tiles) this cost model puts the largest
+// workgroup load at 1.01x the mean, against 1.19x for a round-robin walk.
 template <typename G>
 int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*>& probs_in,
                  int blocks_per_cu, cosma_grouped_plan_s::Bucket* out) {
@@ -462,6 +491,7 @@ int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*
                    [](const cosma_group_problem* a, const cosma_group_problem* b) { return a->k > b->k; });
   std::vector<cosma::GroupEntry<G>> table;
   table.reserve(probs.size());
+  std::vector<int64_t> cost;  // per problem: cost of each of its tiles
   int64_t tiles = 0;
   for (const cosma_group_problem* q : probs) {
     cosma::GroupEntry<G> e{};
@@ -481,22 +511,56 @@ int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*
     tiles += (int64_t)cdiv(q->m, td.bm) * e.tiles_n;
     if (tiles > 0x7fffffff) return COSMA_STATUS_INVALID_VALUE;
     table.push_back(e);
+    cost.push_back(cdiv(q->k, td.bk) + kTileOverheadSteps);
   }
   out->cfg = cfg;
   out->count = (int)table.size();
   out->tiles = (int)tiles;
   const int occ = blocks_per_cu > 0 ? blocks_per_cu : cosma::grouped_occupancy(mode, cfg);
-  int64_t blocks = (int64_t)occ * kCUs;
-  if (blocks > tiles) blocks = tiles;
-  out->blocks = (int)(blocks > 0 ? blocks : 1);
+  int64_t blocks64 = (int64_t)occ * kCUs;
+  if (blocks64 > tiles) blocks64 = tiles;
+  const int B = (int)(blocks64 > 0 ? blocks64 : 1);
+  out->blocks = B;
+
+  // Longest-processing-time-first schedule (see above).
+  std::vector<std::vector<int>> lists(B);
+  using Slot = std::pair<int64_t, int>;  // (load, logical workgroup)
+  std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+  for (int l = 0; l < B; ++l) heap.push(Slot{0, l});
+  for (size_t i = 0; i < table.size(); ++i) {
+    const int64_t first = table[i].tile_start;
+    const int64_t last = i + 1 < table.size() ? table[i + 1].tile_start : tiles;
+    for (int64_t t = first; t < last; ++t) {
+      Slot s = heap.top();
+      heap.pop();
+      lists[s.second].push_back((int)t);
+      heap.push(Slot{s.first + cost[i], s.second});
+    }
+  }
+  const int per_xcd = B % 8 == 0 ? B / 8 : 0;
+  std::vector<int> sched((size_t)B + 1 + (size_t)tiles);
+  std::vector<int> of_phys(B);  // logical workgroup run by physical id
+  for (int l = 0; l < B; ++l) of_phys[per_xcd ? (l % per_xcd) * 8 + l / per_xcd : l] = l;
+  int pos = 0;
+  for (int w = 0; w < B; ++w) {
+    sched[w] = pos;
+    for (int t : lists[of_phys[w]]) sched[(size_t)B + 1 + pos++] = t;
+  }
+  sched[B] = pos;
+
+  const size_t tbytes = (table.size() * sizeof(cosma::GroupEntry<G>) + 255) / 256 * 256;
+  const size_t sbytes = sched.size() * sizeof(int);
   out->table = nullptr;
-  const size_t bytes = table.size() * sizeof(cosma::GroupEntry<G>);
-  hipError_t e = hipMalloc(&out->table, bytes);
+  out->sched = nullptr;
+  hipError_t e = hipMalloc(&out->table, tbytes + sbytes);
   if (e != hipSuccess) {
     out->table = nullptr;
     return (int)e;
   }
-  e = hipMemcpy(out->table, table.data(), bytes, hipMemcpyHostToDevice);
+  out->sched = reinterpret_cast<const int*>(static_cast<char*>(out->table) + tbytes);
+  e = hipMemcpy(out->table, table.data(), table.size() * sizeof(cosma::GroupEntry<G>), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(const_cast<int*>(out->sched), sched.data(), sbytes, hipMemcpyHostToDevice);
   return (int)e;
 }
 
@@ -512,7 +576,7 @@ void free_plan(cosma_grouped_plan_s* p) {
 int launch_plan(cosma_grouped_plan_s* p, hipStream_t s) {
   for (auto& b : p->buckets) {
     if (b.tiles == 0) continue;
-    hipError_t e = cosma::run_grouped(p->mode, b.cfg, b.table, b.count, b.tiles, b.blocks, s);
+    hipError_t e = cosma::run_grouped(p->mode, b.cfg, b.table, b.count, b.sched, b.blocks, s);
     if (e != hipSuccess) return (int)e;
   }
   return 0;
@@ -529,6 +593,7 @@ int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* pr
   if (mode < 0 || mode >= kNumModes || count < 0 || (count > 0 && !problems))
     return COSMA_STATUS_INVALID_VALUE;
   std::vector<std::vector<const cosma_group_problem*>> by_cfg(kNumCfg[mode]);
+  const int default_cfg = group_default_cfg(mode, count, problems);
   double flops = 0.0;
   for (int i = 0; i < count; ++i) {
     const cosma_group_problem& q = problems[i];
@@ -541,7 +606,7 @@ int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* pr
     if (st != 0) return st;
     if (q.m == 0 || q.n == 0) continue;
     if (q.k > 0 && q.alpha != 0.0 && ((uintptr_t)q.c == 0)) return COSMA_STATUS_INVALID_VALUE;
-    int cfg = (tile_cfgs && tile_cfgs[i] >= 0) ? tile_cfgs[i] : group_cfg(mode, q.m, q.n);
+    int cfg = (tile_cfgs && tile_cfgs[i] >= 0) ? tile_cfgs[i] : default_cfg;
     if (cfg >= kNumCfg[mode]) return COSMA_STATUS_INVALID_VALUE;
     if (is_planar(mode, cfg)) return COSMA_STATUS_NOT_SUPPORTED;
     by_cfg[cfg].push_back(&q);
@@ -597,7 +662,7 @@ int cosma_grouped_plan_run_graph(cosma_grouped_plan plan, void* stream) {
       if (b > 0) st = (int)hipStreamWaitEvent(bs, ev[0], 0);
       const auto& k = plan->buckets[b];
       if (st == 0 && k.tiles > 0)
-        st = (int)cosma::run_grouped(plan->mode, k.cfg, k.table, k.count, k.tiles, k.blocks, bs);
+        st = (int)cosma::run_grouped(plan->mode, k.cfg, k.table, k.count, k.sched, k.blocks, bs);
       if (st == 0 && b > 0) st = (int)hipEventRecord(ev[b], bs);
     }
     for (int b = 1; b < nb && st == 0; ++b) st = (int)hipStreamWaitEvent(cs, ev[b], 0);  // join

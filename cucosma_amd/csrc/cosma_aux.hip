@@ -146,24 +146,24 @@ extern const int kBF16x3NumTiles;
 extern const TileInfo kBF16x4Tiles[];
 extern const int kBF16x4NumTiles;
 
-hipError_t run_grouped_f32(int cfg, const void* table, int count, int total, int blocks, hipStream_t s);
-hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int blocks, hipStream_t s);
-hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, int total, int blocks,
+hipError_t run_grouped_f32(int cfg, const void* table, int count, const int* sched, int blocks, hipStream_t s);
+hipError_t run_grouped_f64(int cfg, const void* table, int count, const int* sched, int blocks, hipStream_t s);
+hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, const int* sched, int blocks,
                               hipStream_t s);
 int grouped_occupancy_f32(int cfg);
 int grouped_occupancy_f64(int cfg);
 int grouped_occupancy_bf16x3(int cfg);
-hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, int total, int blocks,
+hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, const int* sched, int blocks,
                               hipStream_t s);
 int grouped_occupancy_bf16x4(int cfg);
 
-hipError_t run_grouped(int mode, int cfg, const void* table, int count, int total_tiles, int blocks,
+hipError_t run_grouped(int mode, int cfg, const void* table, int count, const int* sched, int blocks,
                        hipStream_t s) {
   switch (mode) {
-    case kF32: return run_grouped_f32(cfg, table, count, total_tiles, blocks, s);
-    case kF64: return run_grouped_f64(cfg, table, count, total_tiles, blocks, s);
-    case kBF16x3: return run_grouped_bf16x3(cfg, table, count, total_tiles, blocks, s);
-    case kBF16x4: return run_grouped_bf16x4(cfg, table, count, total_tiles, blocks, s);
+    case kF32: return run_grouped_f32(cfg, table, count, sched, blocks, s);
+    case kF64: return run_grouped_f64(cfg, table, count, sched, blocks, s);
+    case kBF16x3: return run_grouped_bf16x3(cfg, table, count, sched, blocks, s);
+    case kBF16x4: return run_grouped_bf16x4(cfg, table, count, sched, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }

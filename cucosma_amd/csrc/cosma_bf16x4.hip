@@ -35,16 +35,16 @@ hipError_t run_gemm_bf16x4(const Problem& p) {
   }
 }
 
-hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, int total, int blocks,
+hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, const int* sched, int blocks,
                          hipStream_t s) {
   switch (cfg) {
-    case 0: return B4C0::grouped(table, count, total, blocks, s);
-    case 1: return B4C1::grouped(table, count, total, blocks, s);
-    case 2: return B4C2::grouped(table, count, total, blocks, s);
-    case 3: return B4C3::grouped(table, count, total, blocks, s);
-    case 4: return B4C4::grouped(table, count, total, blocks, s);
-    case 5: return B4C5::grouped(table, count, total, blocks, s);
-    case 6: return B4C6::grouped(table, count, total, blocks, s);
+    case 0: return B4C0::grouped(table, count, sched, blocks, s);
+    case 1: return B4C1::grouped(table, count, sched, blocks, s);
+    case 2: return B4C2::grouped(table, count, sched, blocks, s);
+    case 3: return B4C3::grouped(table, count, sched, blocks, s);
+    case 4: return B4C4::grouped(table, count, sched, blocks, s);
+    case 5: return B4C5::grouped(table, count, sched, blocks, s);
+    case 6: return B4C6::grouped(table, count, sched, blocks, s);
     case 7: return hipErrorNotSupported;  // pre-split: no grouped launch
     default: return hipErrorInvalidValue;
   }

@@ -39,17 +39,17 @@ hipError_t run_gemm_f64(const Problem& p) {
   }
 }
 
-hipError_t run_grouped_f64(int cfg, const void* table, int count, int total, int blocks,
+hipError_t run_grouped_f64(int cfg, const void* table, int count, const int* sched, int blocks,
                          hipStream_t s) {
   switch (cfg) {
-    case 0: return F64C0::grouped(table, count, total, blocks, s);
-    case 1: return F64C1::grouped(table, count, total, blocks, s);
-    case 2: return F64C2::grouped(table, count, total, blocks, s);
-    case 3: return F64C3::grouped(table, count, total, blocks, s);
-    case 4: return F64C4::grouped(table, count, total, blocks, s);
-    case 5: return F64C5::grouped(table, count, total, blocks, s);
-    case 6: return F64C6::grouped(table, count, total, blocks, s);
-    case 7: return F64C7::grouped(table, count, total, blocks, s);
+    case 0: return F64C0::grouped(table, count, sched, blocks, s);
+    case 1: return F64C1::grouped(table, count, sched, blocks, s);
+    case 2: return F64C2::grouped(table, count, sched, blocks, s);
+    case 3: return F64C3::grouped(table, count, sched, blocks, s);
+    case 4: return F64C4::grouped(table, count, sched, blocks, s);
+    case 5: return F64C5::grouped(table, count, sched, blocks, s);
+    case 6: return F64C6::grouped(table, count, sched, blocks, s);
+    case 7: return F64C7::grouped(table, count, sched, blocks, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -83,8 +83,9 @@ hipError_t run_gemm_bf16x3(const Problem& p);
 hipError_t run_gemm_bf16x4(const Problem& p);
 
 // Grouped persistent launch of `blocks` workgroups over `count` table
-// entries (device memory) holding `total_tiles` tiles of tile config cfg.
-hipError_t run_grouped(int mode, int cfg, const void* table, int count, int total_tiles, int blocks,
+// entries (device memory) of tile config cfg; sched (device memory) holds
+// blocks + 1 offsets and then the tile ids of each workgroup.
+hipError_t run_grouped(int mode, int cfg, const void* table, int count, const int* sched, int blocks,
                        hipStream_t s);
 // Workgroups of the grouped kernel of (mode, cfg) that fit on one CU.
 int grouped_occupancy(int mode, int cfg);

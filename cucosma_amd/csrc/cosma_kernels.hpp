@@ -719,20 +719,24 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
 // ---------------------------------------------------------------------------
 // Grouped GEMM: many independent row-major problems in one persistent launch.
 // The table lists every problem with the index of its first tile in a global
-// tile numbering (tile_start, ascending).  Workgroup w processes tiles
-// w, w + gridDim.x, ... ; each tile is located by binary search in the table
+// tile numbering (tile_start, ascending).  sched holds the host-computed
+// schedule: sched[w] .. sched[w + 1] index the tiles of workgroup w in
+// sched[gridDim.x + 1 + i] (a longest-first assignment that balances the
+// K-steps per workgroup); each tile is located by binary search in the table
 // and computed with the same gemm_tile as the single-problem kernel (split 1,
-// bounds-checked edges).  Tiles are numbered problem by problem, row-major
-// within a problem, so concurrently running workgroups share operand panels.
+// bounds-checked edges), so the schedule never changes a result.
 // ---------------------------------------------------------------------------
 template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, int OCC>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amdgpu_waves_per_eu(OCC)))
     grouped_kernel(const GroupEntry<typename Traits<MODE>::G>* __restrict__ table, int count,
-                   int total_tiles) {
+                   const int* __restrict__ sched) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   using E = typename Traits<MODE>::E;
   __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
-  for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+  const int* order = sched + gridDim.x + 1;
+  const int end = sched[blockIdx.x + 1];
+  for (int i = sched[blockIdx.x]; i < end; ++i) {
+    const int t = order[i];
     int lo = 0, hi = count - 1;  // last entry with tile_start <= t
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;

@@ -21,10 +21,10 @@ struct Tile {
     return hipGetLastError();
   }
 
-  static hipError_t grouped(const void* table, int count, int total, int blocks, hipStream_t s) {
+  static hipError_t grouped(const void* table, int count, const int* sched, int blocks, hipStream_t s) {
     using G = typename Traits<MODE>::G;
     hipLaunchKernelGGL((grouped_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC>), dim3(blocks), dim3(Sh::NT), 0, s,
-                       static_cast<const GroupEntry<G>*>(table), count, total);
+                       static_cast<const GroupEntry<G>*>(table), count, sched);
     return hipGetLastError();
   }
 

@@ -20,7 +20,8 @@ class GroupedGemm:
 
     The plan keeps the operand tensors alive; their storage must not move
     while the plan exists.  tile_cfgs: per-problem tile configuration (-1 =
-    library choice), e.g. from the autotuner.
+    library choice: one configuration for all such problems, picked from the
+    batch's flop-weighted output sizes), e.g. from the autotuner.
     """
 
     def __init__(self, As, Bs, Cs, *, mode: str = "fp32", alpha: float = 1.0, beta: float = 0.0,

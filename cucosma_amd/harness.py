@@ -20,7 +20,9 @@ Configurations:
   dgemm    32768x1024x1024 and 1024x65536x1024, normal(0,1) seed 2 with 1% of
            entries x1e6, C normal(0,1), alpha = beta = 1
   grouped  512 problems, M/N/K log-uniform in [32, 2048] rounded to 32, seed 3:
-           one hipGraph replay of the persistent grouped launch vs one launch per problem
+           one hipGraph replay of the library's default grouped plan vs one
+           launch per problem, and every assignment of 64x64 / 128x128 /
+           large tiles to three output-size classes for comparison
   autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
@@ -216,42 +218,38 @@ def cfg_grouped(cc, torch, args, emit):
         As = [torch.rand((m, k), device="cuda", generator=g) * 2 - 1 for m, n, k in sizes]
         Bs = [torch.rand((k, n), device="cuda", generator=g) * 2 - 1 for m, n, k in sizes]
         Cs = [torch.empty((m, n), device="cuda") for m, n, k in sizes]
-        # Size buckets (largest output side): the tile per bucket is chosen by
-        # timing every combination of the candidate tiles.
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+        def time_plan(plan, reps):
+            plan.run_graph()
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(reps):
+                plan.run_graph()
+            ev[1].record()
+            torch.cuda.synchronize()
+            return ev[0].elapsed_time(ev[1]) / reps
+
+        # The library's default plan (one tile configuration for the batch).
+        plan = cc.GroupedGemm(As, Bs, Cs, mode=mode)
+        ms_graph = time_plan(plan, args.runs)
+        info = plan.info()
+        plan.close()
+        # Against it: every per-size-class assignment of the candidate tiles
+        # (largest output side below 256, below 1024, from 1024).
         edges = (256, 1024)
         bucket = [0 if max(m, n) < edges[0] else (1 if max(m, n) < edges[1] else 2)
                   for m, n, k in sizes]
-        large = {"fp32": 5, "bf16x3": 6}[mode]
+        large = {"fp32": 7, "bf16x3": 6}[mode]
         cand = (1, 0, large)
-        best = None
         tried = []
         for c0 in cand:
             for c1 in cand:
                 for c2 in cand:
-                    cfgs = [(c0, c1, c2)[b] for b in bucket]
-                    plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=cfgs)
-                    plan.run_graph()
-                    torch.cuda.synchronize()
-                    t0 = time.perf_counter()
-                    for _ in range(5):
-                        plan.run_graph()
-                    torch.cuda.synchronize()
-                    t = (time.perf_counter() - t0) / 5
-                    tried.append(((c0, c1, c2), t))
-                    if best is None or t < best[1]:
-                        best = ((c0, c1, c2), t)
-                    plan.close()
-        cfgs = [best[0][b] for b in bucket]
-        plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=cfgs)
-        plan.run_graph()
-        torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        ev[0].record()
-        for _ in range(args.runs):
-            plan.run_graph()
-        ev[1].record()
-        torch.cuda.synchronize()
-        ms_graph = ev[0].elapsed_time(ev[1]) / args.runs
+                    p = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=[(c0, c1, c2)[b] for b in bucket])
+                    tried.append(((c0, c1, c2), time_plan(p, 10)))
+                    p.close()
+        best = min(tried, key=lambda x: x[1])
         # one launch per problem (library heuristic per shape)
         for A, B, Cm in zip(As, Bs, Cs):
             cc.gemm(A, B, Cm, mode=mode)
@@ -264,15 +262,14 @@ def cfg_grouped(cc, torch, args, emit):
         ev[1].record()
         torch.cuda.synchronize()
         ms_loop = ev[0].elapsed_time(ev[1]) / reps
-        info = plan.info()
         emit(dict(config="grouped", mode=mode, problems=len(sizes), gflop=flops / 1e9,
-                  bucket_edges=list(edges), bucket_tile_cfgs=list(best[0]), buckets=info["buckets"],
-                  searched={",".join(map(str, c)): round(t * 1e3, 4) for c, t in tried},
+                  buckets=info["buckets"], bucket_edges=list(edges),
+                  searched={",".join(map(str, c)): round(t, 4) for c, t in tried},
+                  best_searched=",".join(map(str, best[0])), ms_best_searched=best[1],
                   ms_graph=ms_graph, tflops_graph=flops / ms_graph / 1e9,
                   ms_per_problem_launches=ms_loop, tflops_per_problem_launches=flops / ms_loop / 1e9,
                   speedup=ms_loop / ms_graph,
                   pct_peak=100 * flops / ms_graph / 1e9 / PEAK[mode]))
-        plan.close()
         del As, Bs, Cs
 
 

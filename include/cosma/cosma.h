@@ -255,10 +255,13 @@ typedef struct cosma_group_problem {
 typedef struct cosma_grouped_plan_s *cosma_grouped_plan;
 
 /* Validate the problems (same contract as cosma_gemm_ex, row-major only),
- * bucket them by tile configuration (tile_cfgs[i] for problem i, or the
- * library heuristic when tile_cfgs is NULL or tile_cfgs[i] < 0), and upload
- * one device table per bucket.  Allocates and synchronises; do not call
- * while capturing.  blocks_per_cu <= 0 uses the kernel's occupancy. */
+ * bucket them by tile configuration (tile_cfgs[i] for problem i; problems
+ * without one -- tile_cfgs NULL or tile_cfgs[i] < 0 -- share one default
+ * configuration chosen from the batch's flop-weighted output sizes), and
+ * upload one device table and workgroup schedule (longest tiles first, to
+ * the least-loaded workgroup) per bucket.  Allocates and synchronises; do
+ * not call while capturing.  blocks_per_cu <= 0 uses the kernel's
+ * occupancy. */
 int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem *problems,
                               const int *tile_cfgs, int blocks_per_cu, cosma_grouped_plan *plan);
 /* One persistent-kernel launch per bucket on stream (no allocation, no copy:

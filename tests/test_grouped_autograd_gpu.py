@@ -91,6 +91,29 @@ def test_grouped_empty_and_explicit_tiles(cc, dev):
 
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16x4", "fp64"])
+def test_grouped_default_plan_is_one_launch(cc, dev, mode):
+    """Without tile_cfgs the whole batch shares one tile configuration (one
+    persistent launch): the mode's large tile when outputs with a side of at
+    least 256 hold half the flops, else 128x128, else 64x64."""
+    from cucosma_amd.harness import grouped_sizes
+
+    dt = torch.float64 if mode == "fp64" else torch.float32
+    # the large tiles: FP32 256x128 (16 waves), FP64 128x128 (8 waves), BF16 256x256 on the fly
+    large = {"fp32": 7, "fp64": 7, "bf16x3": 6, "bf16x4": 3}[mode]
+    for sizes, want in ((grouped_sizes(), large), ([(96, 160, 64)] * 3 + [(40, 32, 512)], 0),
+                        ([(64, 64, 64), (32, 96, 128)], 1)):
+        As = [torch.empty((m, k), device=dev, dtype=dt) for m, n, k in sizes]
+        Bs = [torch.empty((k, n), device=dev, dtype=dt) for m, n, k in sizes]
+        Cs = [torch.empty((m, n), device=dev, dtype=dt) for m, n, k in sizes]
+        plan = cc.GroupedGemm(As, Bs, Cs, mode=mode)
+        b = plan.info()["buckets"]
+        assert len(b) == 1 and b[0]["tile_cfg"] == want and b[0]["problems"] == len(sizes), b
+        assert 1 <= b[0]["blocks"] <= b[0]["tiles"]
+        plan.close()
+        del As, Bs, Cs
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16x4", "fp64"])
 def test_grouped_bitwise_equals_single_launches(cc, dev, mode):
     """Every tile configuration in one plan: each problem's result is bitwise
     the single-launch result of the same tile configuration without split-K
