@@ -243,13 +243,13 @@ def cfg_grouped(cc, torch, args, emit):
         large = {"fp32": 7, "bf16x3": 6}[mode]
         cand = (1, 0, large)
         tried = []
-        for c0 in cand:
+        for c0 in () if args.no_search else cand:
             for c1 in cand:
                 for c2 in cand:
                     p = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=[(c0, c1, c2)[b] for b in bucket])
                     tried.append(((c0, c1, c2), time_plan(p, 10)))
                     p.close()
-        best = min(tried, key=lambda x: x[1])
+        best = min(tried, key=lambda x: x[1]) if tried else ((), float("nan"))
         # one launch per problem (library heuristic per shape)
         for A, B, Cm in zip(As, Bs, Cs):
             cc.gemm(A, B, Cm, mode=mode)
@@ -318,6 +318,8 @@ def main(argv=None) -> int:
     ap.add_argument("--csv-dir", help="write <M>x<N>x<K><name>.csv files (reference format)")
     ap.add_argument("--tune-db", help="apply this autotuner DB before running")
     ap.add_argument("--out", help="append JSON lines here")
+    ap.add_argument("--no-search", action="store_true",
+                    help="grouped: time only the default plan (skip the tile-assignment search)")
     args = ap.parse_args(argv)
 
     import torch
