@@ -84,7 +84,12 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    (planar_cfg): the split pass costs (M+N)*K*8 bytes of HBM traffic, and
 //    the GEMM without the in-loop split is 16% faster with 8 waves, 22% with
 //    16 (8192^3: 2.35 ms in total against 2.64 for cfg 6; 4096^3 and
-//    2048x8192x2048 lose to cfg 6, which they keep);
+//    2048x8192x2048 lose to cfg 6, which they keep).  The pre-split tile is
+//    also taken when its 256x256 tiles number between half the CUs and all
+//    of them (one workgroup per CU, some CUs idle): there the 16-wave kernel's
+//    faster K loop outweighs the split pass (BF16x3 +2% to +17% over nine
+//    shapes from 2048x4096x4096 to 3584^2x4096, row-major and A transposed;
+//    BF16x4 -3% to +3% row-major, +5% to +13% with A transposed);
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    in 16 waves of 64x32 with the next stage held in registers (four waves
 //    per SIMD; 3-5% above the 8-wave version, which itself matched the
@@ -135,8 +140,9 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     const int64_t covered = (int64_t)lt * kDims[mode][cfg].bm * kDims[mode][cfg].bn;
     const int split = lt >= kCUs / 2 ? 1 : clamp_split(kCUs / lt, k, 512, kCUs);
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2) {
-      // Large outputs: split the operands once and copy planes into LDS.
-      if (split == 1 && lt >= kCUs && (int64_t)m * n >= 3072 * ((int64_t)m + n))
+      // Large outputs, and outputs whose tiles leave CUs idle: split the
+      // operands once and copy planes into LDS.
+      if (split == 1 && (lt < kCUs || (int64_t)m * n >= 3072 * ((int64_t)m + n)))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }

@@ -268,6 +268,25 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
         assert np.all(np.abs(outs[0] - ref) <= 2 * (K + 2) * 2.0**-24 * 1.25 * absprod + 1e-30)
 
 
+@pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
+def test_heuristic_presplit_for_underfilled_tiles(cc, dev, mode):
+    """Outputs whose 256x256 tiles fill between half the CUs and all of them
+    take the pre-split configuration by default (library workspace for the
+    planes); the result is bitwise the on-the-fly 256x256 tile's."""
+    planars, same = PRESPLIT[mode]
+    M, N, K = 3072, 3072, 256
+    assert cc.heuristic(cc._lib.MODES[mode], M, N, K) == (planars[-1], 1)
+    assert cc.heuristic(cc._lib.MODES[mode], 4096, 4096, 4096)[0] == same
+    rng = np.random.default_rng(403)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    for ta in (False, True):
+        Ad, Bd = _t(A.T if ta else A, dev), _t(B, dev)
+        got = cc.gemm(Ad, Bd, mode=mode, trans_a=ta).cpu().numpy()
+        want = cc.gemm(Ad, Bd, mode=mode, trans_a=ta, tile_cfg=same, split_k=1).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), ta
+
+
 @pytest.mark.parametrize("ta,tb", LAYOUTS)
 def test_presplit_caller_planes(cc, dev, ta, tb):
     """Operands split once by split_operand and passed as planes give the
