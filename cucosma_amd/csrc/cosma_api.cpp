@@ -98,8 +98,9 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    ping-pong copies; a 16-wave version of it ran 12% slower).  FP32 splits K in two when
 //    the last round of workgroups would be at most half full (6144^3: 4.5
 //    rounds of 256 become 9; +7%);
-//  - deep-K outputs (few tiles, K >= 16384): FP32 splits K over 256x128 with
-//    ping-pong copies until every CU holds one workgroup; FP64, and outputs
+//  - deep-K outputs (few tiles, K >= 16384): FP32 splits K over the 16-wave
+//    256x128 tile until every CU holds one workgroup (0-6% above the 8-wave
+//    ping-pong version of the tile on four deep shapes); FP64, and outputs
 //    smaller than the large tile, split a 128-row tile with a long K chunk
 //    (128x64x32, or 128x128x64 for BF16) until every CU holds two (128x64) or
 //    one (128x128x64) workgroups;
@@ -123,7 +124,7 @@ constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
 // with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
 constexpr int deep_cfg(int mode, int m, int n) {
-  const int big = mode == 0 ? 6 : (mode == 1 ? 2 : (mode == 2 ? 6 : 3));
+  const int big = mode == 0 ? 7 : (mode == 1 ? 2 : (mode == 2 ? 6 : 3));
   if (m >= kDims[mode][big].bm && n >= kDims[mode][big].bn) return big;
   return split_mode(mode) ? 5 : 2;
 }
