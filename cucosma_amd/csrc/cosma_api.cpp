@@ -56,10 +56,10 @@ constexpr TileDims kDims[4][kMaxCfg] = {
      {256, 256, 32}, {256, 256, 32}, {256, 256, 32}},
     // BF16X4 (same tiles as BF16X3)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}, {256, 256, 32}},
+     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {8, 8, 9, 8};
+constexpr int kNumCfg[kNumModes] = {8, 8, 9, 9};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -114,11 +114,13 @@ struct Choice {
   int cfg, split;
 };
 constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
-// Tile configurations over pre-split operands (cfg >= 7; no grouped launch)
-// and the one the heuristic uses: BF16x3 16 waves of 64x64 (117 VGPRs, four
-// waves per SIMD), BF16x4 8 waves of 128x64 (its 16-wave version spills).
-constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && cfg >= 7; }
-constexpr int planar_cfg(int mode) { return mode == 2 ? 8 : 7; }
+// Tile configurations over pre-split operands (cfg 7: 8 waves of 128x64,
+// cfg 8: 16 waves of 64x64; no grouped launch) and the one the heuristic
+// uses: 16 waves, four per SIMD (BF16x3 124 VGPRs; BF16x4 128, its per-k-step
+// temporaries taken two columns at a time; 8192^3 BF16x4 371 TFLOP/s per
+// call against 352 with 8 waves, 3072^3 and 6144^3 +12-16%).
+constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && (cfg == 7 || cfg == 8); }
+constexpr int planar_cfg(int mode) { (void)mode; return 8; }
 constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile

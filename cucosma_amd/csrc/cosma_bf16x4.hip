@@ -4,7 +4,9 @@
 // copy between the MFMAs (PIPE 1), which leaves room for the per-k-step
 // temporaries without spilling; cfg 6 (ping-pong) spills and is kept only
 // for completeness of the table; cfg 7 is the pre-split 256x256 tile (as in
-// cosma_bf16x3.hip).
+// cosma_bf16x3.hip); cfg 8 is cfg 7 with 16 waves of 64x64 (four waves per
+// SIMD; its per-k-step temporaries are taken two columns at a time to fit in
+// 128 registers).
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -17,9 +19,10 @@ using B4C4 = Tile<kBF16x4, 256, 128, 32, 128, 64>;
 using B4C5 = Tile<kBF16x4, 128, 128, 64, 64, 64>;
 using B4C6 = Tile<kBF16x4, 256, 256, 32, 128, 64, 4>;
 using B4C7 = Planar<kBF16x4, 256, 256, 128, 64>;
+using B4C8 = Planar<kBF16x4, 256, 256, 64, 64>;
 
-extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info(), B4C6::info(), B4C7::info()};
-extern const int kBF16x4NumTiles = 8;
+extern const TileInfo kBF16x4Tiles[] = {B4C0::info(), B4C1::info(), B4C2::info(), B4C3::info(), B4C4::info(), B4C5::info(), B4C6::info(), B4C7::info(), B4C8::info()};
+extern const int kBF16x4NumTiles = 9;
 
 hipError_t run_gemm_bf16x4(const Problem& p) {
   switch (p.cfg) {
@@ -31,6 +34,7 @@ hipError_t run_gemm_bf16x4(const Problem& p) {
     case 5: return B4C5::run(p);
     case 6: return B4C6::run(p);
     case 7: return B4C7::run(p);
+    case 8: return B4C8::run(p);
     default: return hipErrorInvalidValue;
   }
 }
@@ -46,6 +50,7 @@ hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, const int* 
     case 5: return B4C5::grouped(table, count, sched, blocks, s);
     case 6: return B4C6::grouped(table, count, sched, blocks, s);
     case 7: return hipErrorNotSupported;  // pre-split: no grouped launch
+    case 8: return hipErrorNotSupported;
     default: return hipErrorInvalidValue;
   }
 }
@@ -73,6 +78,7 @@ int gemm_occupancy_bf16x4(int cfg, int ta, int tb, bool edge) {
     case 5: return B4C5::occupancy(ta, tb, edge);
     case 6: return B4C6::occupancy(ta, tb, edge);
     case 7: return B4C7::occupancy(ta, tb, edge);
+    case 8: return B4C8::occupancy(ta, tb, edge);
     default: return 0;
   }
 }

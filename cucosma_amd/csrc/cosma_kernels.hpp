@@ -376,30 +376,36 @@ struct TileShape {
 
 // The products of one sub-tile row of the split modes for one 32-deep
 // k-step: smallest terms first, then the leading hi*hi term, the product
-// index outermost so TN independent MFMAs separate two updates of one
+// index outermost so independent MFMAs separate two updates of one
 // accumulator.  kBF16x3 chains the three products into the running
 // accumulator; kBF16x4 sums its four products of this k-step in a zeroed
 // temporary and adds that with one round-to-nearest VALU add, so the large
-// running sum is rounded once per k-step.
-template <int MODE, int TN>
+// running sum is rounded once per k-step.  G: kBF16x4 columns whose
+// temporaries are live at once (TN: all; fewer saves registers for kernels
+// with four waves per SIMD).  Every column's sum is the same for any G.
+template <int MODE, int TN, int G = TN>
 __device__ __forceinline__ void split_products(f32x4 (&row)[TN], const bf16x8& ah, const bf16x8& al,
                                                const bf16x8 (&bh)[TN], const bf16x8 (&bl)[TN]) {
   if constexpr (MODE == kBF16x4) {
-    f32x4 t[TN];
+    static_assert(TN % G == 0, "whole column groups");
 #pragma unroll
-    for (int j = 0; j < TN; ++j) t[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j0 = 0; j0 < TN; j0 += G) {
+      f32x4 t[G];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j], t[j], 0, 0, 0);
+      for (int j = 0; j < G; ++j) t[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], t[j], 0, 0, 0);
+      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j0 + j], t[j], 0, 0, 0);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], t[j], 0, 0, 0);
+      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j0 + j], t[j], 0, 0, 0);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], t[j], 0, 0, 0);
+      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j0 + j], t[j], 0, 0, 0);
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j0 + j], t[j], 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) row[j][r] += t[j][r];
+      for (int j = 0; j < G; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) row[j0 + j][r] += t[j][r];
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], row[j], 0, 0, 0);
@@ -911,30 +917,54 @@ __device__ __forceinline__ void planar_tile(const PlanarParams<MODE>& pp, __bf16
   const int cbeg = z * (p.kslice / BK);
   const int cend = min(pp.Kp / BK, cbeg + p.kslice / BK);
   const int nk = cend > cbeg ? cend - cbeg : 0;
+  // Row r = 8 * block + lane / 8 of a block loads chunk c of the layout; the
+  // chunk permutation depends on (r / 2) % 4 = (lane / 16) % 4 only, so all
+  // blocks of a lane share c.  Without edges every block's source is a
+  // wave-uniform row base (scalar registers) plus one per-lane byte offset
+  // shared by all blocks of both operands (rowlen * 2 * 8 < 2^32 is checked
+  // by the launcher); 64-bit per-lane pointers per block would push the
+  // 16-wave BF16x4 kernel past its 128 registers.  With edges each block
+  // clamps its rows to the operand, so each keeps its own pointer.
   const int rin = lane >> 3;
-  const __bf16* srcA[PA];
-  const __bf16* srcB[PB];
+  const int c = (lane & 7) ^ (((rin >> 1) & 3) << 1);  // inverse of LdsLayout (an involution)
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t loff = (uint32_t)(((int64_t)rin * rowlen + c * 8) * (int64_t)sizeof(__bf16));
+  const __bf16* srcA[EDGE ? PA : 1];
+  const __bf16* srcB[EDGE ? PB : 1];
+  if constexpr (EDGE) {
 #pragma unroll
-  for (int j = 0; j < PA; ++j) {
-    const int r = 8 * (wave * PA + j) + rin;
-    const int c = (lane & 7) ^ (((r >> 1) & 3) << 1);  // inverse of LdsLayout (an involution)
-    srcA[j] = pp.Ap + (int64_t)min(m0 + r, p.M - 1) * rowlen + (int64_t)cbeg * ROWE + c * 8;
-  }
+    for (int j = 0; j < PA; ++j)
+      srcA[j] = pp.Ap + (int64_t)min(m0 + 8 * (wv * PA + j) + rin, p.M - 1) * rowlen + (int64_t)cbeg * ROWE + c * 8;
 #pragma unroll
-  for (int j = 0; j < PB; ++j) {
-    const int r = 8 * (wave * PB + j) + rin;
-    const int c = (lane & 7) ^ (((r >> 1) & 3) << 1);
-    srcB[j] = pp.Bp + (int64_t)min(n0 + r, p.N - 1) * rowlen + (int64_t)cbeg * ROWE + c * 8;
+    for (int j = 0; j < PB; ++j)
+      srcB[j] = pp.Bp + (int64_t)min(n0 + 8 * (wv * PB + j) + rin, p.N - 1) * rowlen + (int64_t)cbeg * ROWE + c * 8;
   }
   auto issue = [&](int it, int s) {
     __bf16* st = lds + s * STAGE;
 #pragma unroll
-    for (int j = 0; j < PA; ++j)
-      __builtin_amdgcn_global_load_lds(srcA[j] + it * ROWE, st + (wave * PA + j) * 8 * ROWE, 16, 0, 0);
+    for (int j = 0; j < PA; ++j) {
+      const __bf16* g;
+      if constexpr (EDGE) {
+        g = srcA[j] + it * ROWE;
+      } else {
+        const char* base = reinterpret_cast<const char*>(pp.Ap + (int64_t)(m0 + 8 * (wv * PA + j)) * rowlen +
+                                                         (int64_t)(cbeg + it) * ROWE);
+        g = reinterpret_cast<const __bf16*>(base + loff);
+      }
+      __builtin_amdgcn_global_load_lds(g, st + (wv * PA + j) * 8 * ROWE, 16, 0, 0);
+    }
 #pragma unroll
-    for (int j = 0; j < PB; ++j)
-      __builtin_amdgcn_global_load_lds(srcB[j] + it * ROWE, st + BM * ROWE + (wave * PB + j) * 8 * ROWE, 16,
-                                       0, 0);
+    for (int j = 0; j < PB; ++j) {
+      const __bf16* g;
+      if constexpr (EDGE) {
+        g = srcB[j] + it * ROWE;
+      } else {
+        const char* base = reinterpret_cast<const char*>(pp.Bp + (int64_t)(n0 + 8 * (wv * PB + j)) * rowlen +
+                                                         (int64_t)(cbeg + it) * ROWE);
+        g = reinterpret_cast<const __bf16*>(base + loff);
+      }
+      __builtin_amdgcn_global_load_lds(g, st + BM * ROWE + (wv * PB + j) * 8 * ROWE, 16, 0, 0);
+    }
   };
 
   f32x4 acc[TM][TN];
@@ -968,7 +998,7 @@ __device__ __forceinline__ void planar_tile(const PlanarParams<MODE>& pp, __bf16
       const int row = wm0 + i * 16 + r16;
       const bf16x8 ah = *reinterpret_cast<const bf16x8*>(sa + L::off(row, g * 8));
       const bf16x8 al = *reinterpret_cast<const bf16x8*>(sa + L::off(row, BK + g * 8));
-      split_products<MODE, TN>(acc[i], ah, al, bh, bl);
+      split_products<MODE, TN, (NT >= 1024 && TN > 2) ? 2 : TN>(acc[i], ah, al, bh, bl);
     }
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();

@@ -187,7 +187,9 @@ struct Planar {
     pp.Kp = Kp;
     const int64_t nblk64 = (int64_t)gp.tiles_m * gp.tiles_n * pr.split;
     if (nblk64 <= 0 || nblk64 > 0x7fffffff) return hipErrorInvalidValue;
-    const bool edge = (pr.M % BM) || (pr.N % BN);
+    // The unchecked instantiation addresses a lane's rows with a 32-bit byte
+    // offset of up to 8 plane rows.
+    const bool edge = (pr.M % BM) || (pr.N % BN) || (int64_t)Kp * 2 * 2 * 8 >= ((int64_t)1 << 32);
     if (edge)
       hipLaunchKernelGGL((planar_kernel<MODE, BM, BN, WM, WN, true>), dim3((unsigned)nblk64), dim3(kThreads), 0,
                          pr.stream, pp);
