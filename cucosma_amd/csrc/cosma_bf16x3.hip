@@ -10,8 +10,9 @@
 // cfg 6: cfg 3 with ping-pong copies (PIPE 4): the large-shape tile
 // cfg 7: 256x256, 8 waves of 128x64 over pre-split operands (split pass, then
 //        direct-to-LDS loads; Planar in cosma_launch.hpp); not for grouped plans
-// cfg 8: cfg 7 with 16 waves of 64x64 (four waves per SIMD): the large-output
-//        configuration (8192^3: planar kernel 508 TFLOP/s against 479 for cfg 7)
+// cfg 8: cfg 7 with 16 waves of 64x64 (four waves per SIMD): the pre-split
+//        configuration the heuristic uses (one box, 8192^3: planar kernels 504
+//        and 507 TFLOP/s for cfg 8 and 7; 3072^3: 317 against 301 per call)
 #include "cosma_launch.hpp"
 
 namespace cosma {
