@@ -820,53 +820,46 @@ __global__ void __launch_bounds__(256)
   *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
 }
 
-// Operand element (r, k) at X[k * ld + r] (r-contiguous): transposed through
-// LDS in tiles of 64 rows x 64 k, so reads (16 bytes per lane along r when
-// the rows allow it) and writes (whole 32-k chunks of a row) are contiguous.
-// 1-D grid: nrb = ceil(R / 64) row blocks times ceil(Kp / 64) k blocks.
+// Operand element (r, k) at X[k * ld + r] (r-contiguous): transposed in
+// registers.  A thread loads 4 consecutive r from each of 8 consecutive k
+// rows (one 16-byte load per row when the rows allow it), so for each of its
+// r it holds the 8 k of one hi and one lo 16-byte chunk of the plane row.
+// Lanes: the 4 k groups of a 32-deep chunk fastest, then 16 groups of 4 r, so
+// each load instruction reads 256 contiguous bytes of 4 k rows and each store
+// instruction fills 64-byte halves of 16 plane lines (hi, then lo).  A block
+// of 4 waves covers 256 r x 32 k.  1-D grid: nrb = ceil(R / 256) row blocks
+// times Kp / 32 k chunks.
 template <int MODE>
 __global__ void __launch_bounds__(256)
     split_mnmajor(const float* __restrict__ X, int64_t ld, int R, int K, int Kp, int nrb, int vec,
                   __bf16* __restrict__ out) {
-  __shared__ float tile[64][65];
-  const int r0 = (int)(blockIdx.x % (unsigned)nrb) * 64;
-  const int k0 = (int)(blockIdx.x / (unsigned)nrb) * 64;
-  const int tid = threadIdx.x;
-  const bool full = vec && r0 + 64 <= R;
+  const int lane = threadIdx.x & 63;
+  const int r0 = (int)(blockIdx.x % (unsigned)nrb) * 256 + (threadIdx.x >> 6) * 64 + (lane >> 2) * 4;
+  const int chunk = (int)(blockIdx.x / (unsigned)nrb);
+  const int k0 = chunk * 32 + (lane & 3) * 8;
+  if (r0 >= R) return;
+  const bool full = vec && r0 + 4 <= R;
+  f32x4 v[8];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {  // 64 k-rows x 16 vectors of 4 r
-    const int kk = (tid >> 4) + 16 * i;
-    const int rv = (tid & 15) * 4;
-    const int k = k0 + kk;
-    const float* x = X + (int64_t)k * ld + r0 + rv;
-    f32x4 v;
+  for (int e = 0; e < 8; ++e) {
+    const int k = k0 + e;
+    const float* x = X + (int64_t)k * ld + r0;
     if (full && k < K) {
-      v = *reinterpret_cast<const f32x4*>(x);
+      v[e] = *reinterpret_cast<const f32x4*>(x);
     } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] = (k < K && r0 + rv + e < R) ? x[e] : 0.f;
+      for (int q = 0; q < 4; ++q) v[e][q] = (k < K && r0 + q < R) ? x[q] : 0.f;
     }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) tile[kk][rv + e] = v[e];
   }
-  __syncthreads();
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {  // 64 rows x 8 chunks of 8 k, two per thread
-    const int rr = tid >> 2;
-    const int c = (tid & 3) + 4 * h;
-    const int r = r0 + rr;
-    const int kc = k0 + c * 8;
-    if (r >= R || kc >= Kp) continue;
-    f32x4 v0, v1;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      v0[e] = tile[c * 8 + e][rr];
-      v1[e] = tile[c * 8 + 4 + e][rr];
-    }
+  for (int q = 0; q < 4; ++q) {
+    if (r0 + q >= R) break;
+    const f32x4 a{v[0][q], v[1][q], v[2][q], v[3][q]};
+    const f32x4 b{v[4][q], v[5][q], v[6][q], v[7][q]};
     bf16x4 h0, l0, h1, l1;
-    split_bf16x4(v0, h0, l0);
-    split_bf16x4(v1, h1, l1);
-    __bf16* o = out + (int64_t)r * 2 * Kp + (kc / 32) * 64 + (kc % 32);
+    split_bf16x4(a, h0, l0);
+    split_bf16x4(b, h1, l1);
+    __bf16* o = out + (int64_t)(r0 + q) * 2 * Kp + chunk * 64 + (lane & 3) * 8;
     *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
     *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
   }

@@ -132,8 +132,8 @@ struct Planar {
       const int vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
       hipLaunchKernelGGL(split_kmajor<MODE>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, R, K, Kp, vec, out);
     } else {
-      const int nrb = (R + 63) / 64;
-      const int64_t blocks = (int64_t)nrb * ((Kp + 63) / 64);
+      const int nrb = (R + 255) / 256;
+      const int64_t blocks = (int64_t)nrb * (Kp / 32);
       if (blocks > 0x7fffffff) return hipErrorInvalidValue;
       const int vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
       hipLaunchKernelGGL(split_mnmajor<MODE>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, R, K, Kp, nrb, vec,
