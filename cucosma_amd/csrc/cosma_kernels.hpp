@@ -416,6 +416,28 @@ __device__ __forceinline__ void split_products(f32x4 (&row)[TN], const bf16x8& a
   }
 }
 
+// The products of one sub-tile (one accumulator) for one 32-deep k-step, in
+// the order split_products gives that accumulator: kBF16x3 lo*hi, hi*lo,
+// hi*hi into the running sum; kBF16x4 lo*lo, lo*hi, hi*lo, hi*hi into a
+// zeroed temporary, then one add.
+template <int MODE>
+__device__ __forceinline__ void split_products_one(f32x4& acc, const bf16x8& ah, const bf16x8& al,
+                                                   const bf16x8& bh, const bf16x8& bl) {
+  if constexpr (MODE == kBF16x4) {
+    f32x4 t = f32x4{0.f, 0.f, 0.f, 0.f};
+    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, t, 0, 0, 0);
+    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, t, 0, 0, 0);
+    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, t, 0, 0, 0);
+    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, t, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] += t[r];
+  } else {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+  }
+}
+
 // MFMA sequence over one LDS stage for one wave's TM x TN sub-tiles.
 //
 // PIPE selects how the copy of the next stage (mid(): split + LDS writes of
@@ -979,19 +1001,28 @@ __device__ __forceinline__ void planar_tile(const PlanarParams<MODE>& pp, __bf16
     if (it + 1 < nk) issue(it + 1, cur ^ 1);
     const __bf16* sa = lds + cur * STAGE;
     const __bf16* sb = sa + BM * ROWE;
+    // Sub-tile by sub-tile, row-major: the first row reads each B column's
+    // fragments just before its products, so a wave's first MFMAs wait for
+    // four fragment reads instead of all of them (every wave reads at once
+    // after the barrier); each sub-tile's products are kept together in
+    // program order by a scheduling group.  Each accumulator receives its
+    // products in split_products' order, so results are unchanged.
     bf16x8 bh[TN], bl[TN];
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int row = wn0 + j * 16 + r16;
-      bh[j] = *reinterpret_cast<const bf16x8*>(sb + L::off(row, g * 8));
-      bl[j] = *reinterpret_cast<const bf16x8*>(sb + L::off(row, BK + g * 8));
-    }
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const int row = wm0 + i * 16 + r16;
       const bf16x8 ah = *reinterpret_cast<const bf16x8*>(sa + L::off(row, g * 8));
       const bf16x8 al = *reinterpret_cast<const bf16x8*>(sa + L::off(row, BK + g * 8));
-      split_products<MODE, TN, (NT >= 1024 && TN > 2) ? 2 : TN>(acc[i], ah, al, bh, bl);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        if (i == 0) {
+          const int rowb = wn0 + j * 16 + r16;
+          bh[j] = *reinterpret_cast<const bf16x8*>(sb + L::off(rowb, g * 8));
+          bl[j] = *reinterpret_cast<const bf16x8*>(sb + L::off(rowb, BK + g * 8));
+        }
+        split_products_one<MODE>(acc[i][j], ah, al, bh[j], bl[j]);
+        __builtin_amdgcn_sched_group_barrier(0x008, MODE == kBF16x4 ? 4 : 3, 0);  // this sub-tile's MFMAs
+      }
     }
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
