@@ -493,22 +493,34 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
       const typename T::S* sal = st + Sh::PLANE;
       const typename T::S* sbl = sal + BM * BK;
       Frag bh[TN], bl[TN];
+      if constexpr (PIPE != 0) {
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int off = L::off(wn0 + j * 16 + r16, kof);
-        bh[j] = *reinterpret_cast<const Frag*>(sb + off);
-        bl[j] = *reinterpret_cast<const Frag*>(sbl + off);
+        for (int j = 0; j < TN; ++j) {
+          const int off = L::off(wn0 + j * 16 + r16, kof);
+          bh[j] = *reinterpret_cast<const Frag*>(sb + off);
+          bl[j] = *reinterpret_cast<const Frag*>(sbl + off);
+        }
       }
       if constexpr (PIPE == 0) {
-        Frag ah[TM], al[TM];
+        // Sub-tile by sub-tile, as in planar_tile: A fragments per row, the B
+        // fragments of a column just before its first products, each
+        // sub-tile's products kept together by a scheduling group.
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
           const int off = L::off(wm0 + i * 16 + r16, kof);
-          ah[i] = *reinterpret_cast<const Frag*>(sa + off);
-          al[i] = *reinterpret_cast<const Frag*>(sal + off);
-        }
+          const Frag ah = *reinterpret_cast<const Frag*>(sa + off);
+          const Frag al = *reinterpret_cast<const Frag*>(sal + off);
 #pragma unroll
-        for (int i = 0; i < TM; ++i) split_row(acc[i], ah[i], al[i], bh, bl);
+          for (int j = 0; j < TN; ++j) {
+            if (i == 0) {
+              const int offb = L::off(wn0 + j * 16 + r16, kof);
+              bh[j] = *reinterpret_cast<const Frag*>(sb + offb);
+              bl[j] = *reinterpret_cast<const Frag*>(sbl + offb);
+            }
+            split_products_one<MODE>(acc[i][j], ah, al, bh[j], bl[j]);
+            __builtin_amdgcn_sched_group_barrier(0x008, MODE == kBF16x4 ? 4 : 3, 0);
+          }
+        }
       } else {
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
