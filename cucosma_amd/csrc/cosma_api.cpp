@@ -70,9 +70,9 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // Heuristic tile / split choice, used when neither the caller nor the tuned
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
-//  - BF16x3 / BF16x4: the large tile -- 256x256, 8 waves of 128x64 (BF16x3
-//    with ping-pong copies; BF16x4 with the copy between the MFMAs, which
-//    leaves room for its per-k-step temporaries) -- whenever the output fills
+//  - BF16x3 / BF16x4: the large tile -- 256x256, 8 waves of 128x64 with
+//    ping-pong copies (for BF16x4 0-14% above the copy-between-MFMAs version
+//    on eight shapes, within 3% below on two) -- whenever the output fills
 //    at least half of its tiles and the tiles, with K split into slices of at
 //    least 512, give at least half the CUs a workgroup.  K is split only when
 //    the tiles alone leave more than half the CUs idle, until every CU holds
@@ -121,12 +121,12 @@ constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
 // call against 352 with 8 waves, 3072^3 and 6144^3 +12-16%).
 constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && (cfg == 7 || cfg == 8); }
 constexpr int planar_cfg(int mode) { (void)mode; return 8; }
-constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : (mode == 2 ? 6 : 3)); }
+constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : 6); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
 // with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
 constexpr int deep_cfg(int mode, int m, int n) {
-  const int big = mode == 0 ? 7 : (mode == 1 ? 2 : (mode == 2 ? 6 : 3));
+  const int big = mode == 0 ? 7 : (mode == 1 ? 2 : 6);
   if (m >= kDims[mode][big].bm && n >= kDims[mode][big].bn) return big;
   return split_mode(mode) ? 5 : 2;
 }
@@ -456,7 +456,9 @@ int group_default_cfg(int mode, int count, const cosma_group_problem* problems) 
     if (side >= 256) from256 += f;
     if (side >= 128) from128 += f;
   }
-  if (2.0 * from256 >= all) return large_cfg(mode);
+  // (BF16x4: the copy-between-MFMAs version of the 256x256 tile, 5% above
+  // the ping-pong one in one launch over the 512 problems.)
+  if (2.0 * from256 >= all) return mode == 3 ? 3 : large_cfg(mode);
   return 2.0 * from128 >= all ? 0 : 1;
 }
 

@@ -1,9 +1,10 @@
 // cosma_bf16x4.hip -- split-precision SGEMM with four products (bf16 hi/lo,
 // hi*hi + hi*lo + lo*hi + lo*lo on v_mfma_f32_16x16x32_bf16, FP32
 // accumulation): the tiles of cosma_bf16x3.hip; cfg 3 places the next-stage
-// copy between the MFMAs (PIPE 1), which leaves room for the per-k-step
-// temporaries without spilling; cfg 6 (ping-pong) spills and is kept only
-// for completeness of the table; cfg 7 is the pre-split 256x256 tile (as in
+// copy between the MFMAs (PIPE 1) and is the grouped default; cfg 6
+// (ping-pong) is the large and deep-K tile (its copy-after schedule reads
+// fragments per sub-tile and keeps the per-k-step temporaries of one
+// sub-tile live, so it fits without spilling); cfg 7 is the pre-split 256x256 tile (as in
 // cosma_bf16x3.hip); cfg 8 is cfg 7 with 16 waves of 64x64 (four waves per
 // SIMD; its per-k-step temporaries are taken two columns at a time to fit in
 // 128 registers).

@@ -226,7 +226,7 @@ def test_fused_epilogue(cc, dev, mode, dt, act):
 
 
 # pre-split configurations and the on-the-fly configuration with the same 256x256 tile
-PRESPLIT = {"bf16x3": ((7, 8), 6), "bf16x4": ((7, 8), 3)}
+PRESPLIT = {"bf16x3": ((7, 8), 6), "bf16x4": ((7, 8), 6)}
 
 
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
