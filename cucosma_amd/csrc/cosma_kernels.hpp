@@ -537,16 +537,15 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
       for (int j = 0; j < TN; ++j)
         b[j] = *reinterpret_cast<const Frag*>(sb + L::off(wn0 + j * 16 + r16, kof));
       if constexpr (PIPE == 0) {
-        Frag a[TM];
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
-          a[i] = *reinterpret_cast<const Frag*>(sa + L::off(wm0 + i * 16 + r16, kof));
+        for (int i = 0; i < TM; ++i) {
+          const Frag a = *reinterpret_cast<const Frag*>(sa + L::off(wm0 + i * 16 + r16, kof));
 #pragma unroll
-        for (int s = 0; s < T::STEPS; ++s)
+          for (int s = 0; s < T::STEPS; ++s)
 #pragma unroll
-          for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j) mfma1(acc[i][j], a[i][s], b[j][s]);
+            for (int j = 0; j < TN; ++j) mfma1(acc[i][j], a[s], b[j][s]);
+          __builtin_amdgcn_sched_group_barrier(0x008, T::STEPS * TN, 0);
+        }
       } else {
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
