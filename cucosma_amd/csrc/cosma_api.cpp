@@ -117,7 +117,7 @@ constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
 // Tile configurations over pre-split operands (cfg 7: 8 waves of 128x64,
 // cfg 8: 16 waves of 64x64; no grouped launch) and the one the heuristic
 // uses: 16 waves, four per SIMD (BF16x3 124 VGPRs; BF16x4 128, its per-k-step
-// temporaries taken two columns at a time; 8192^3 BF16x4 371 TFLOP/s per
+// temporary kept for one sub-tile at a time; 8192^3 BF16x4 371 TFLOP/s per
 // call against 352 with 8 waves, 3072^3 and 6144^3 +12-16%).
 constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && (cfg == 7 || cfg == 8); }
 constexpr int planar_cfg(int mode) { (void)mode; return 8; }

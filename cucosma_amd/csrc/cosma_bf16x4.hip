@@ -6,8 +6,8 @@
 // fragments per sub-tile and keeps the per-k-step temporaries of one
 // sub-tile live, so it fits without spilling); cfg 7 is the pre-split 256x256 tile (as in
 // cosma_bf16x3.hip); cfg 8 is cfg 7 with 16 waves of 64x64 (four waves per
-// SIMD; its per-k-step temporaries are taken two columns at a time to fit in
-// 128 registers).
+// SIMD; its per-k-step temporary is kept for one sub-tile at a time, which
+// fits in 128 registers).
 #include "cosma_launch.hpp"
 
 namespace cosma {

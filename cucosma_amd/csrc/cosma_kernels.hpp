@@ -376,36 +376,30 @@ struct TileShape {
 
 // The products of one sub-tile row of the split modes for one 32-deep
 // k-step: smallest terms first, then the leading hi*hi term, the product
-// index outermost so independent MFMAs separate two updates of one
+// index outermost so TN independent MFMAs separate two updates of one
 // accumulator.  kBF16x3 chains the three products into the running
 // accumulator; kBF16x4 sums its four products of this k-step in a zeroed
 // temporary and adds that with one round-to-nearest VALU add, so the large
-// running sum is rounded once per k-step.  G: kBF16x4 columns whose
-// temporaries are live at once (TN: all; fewer saves registers for kernels
-// with four waves per SIMD).  Every column's sum is the same for any G.
-template <int MODE, int TN, int G = TN>
+// running sum is rounded once per k-step.
+template <int MODE, int TN>
 __device__ __forceinline__ void split_products(f32x4 (&row)[TN], const bf16x8& ah, const bf16x8& al,
                                                const bf16x8 (&bh)[TN], const bf16x8 (&bl)[TN]) {
   if constexpr (MODE == kBF16x4) {
-    static_assert(TN % G == 0, "whole column groups");
+    f32x4 t[TN];
 #pragma unroll
-    for (int j0 = 0; j0 < TN; j0 += G) {
-      f32x4 t[G];
+    for (int j = 0; j < TN; ++j) t[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int j = 0; j < G; ++j) t[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j], t[j], 0, 0, 0);
 #pragma unroll
-      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl[j0 + j], t[j], 0, 0, 0);
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], t[j], 0, 0, 0);
 #pragma unroll
-      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j0 + j], t[j], 0, 0, 0);
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], t[j], 0, 0, 0);
 #pragma unroll
-      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j0 + j], t[j], 0, 0, 0);
+    for (int j = 0; j < TN; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], t[j], 0, 0, 0);
 #pragma unroll
-      for (int j = 0; j < G; ++j) t[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j0 + j], t[j], 0, 0, 0);
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int j = 0; j < G; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) row[j0 + j][r] += t[j][r];
-    }
+      for (int r = 0; r < 4; ++r) row[j][r] += t[j][r];
   } else {
 #pragma unroll
     for (int j = 0; j < TN; ++j) row[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], row[j], 0, 0, 0);
