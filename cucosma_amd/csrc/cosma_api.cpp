@@ -104,6 +104,18 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    smaller than the large tile, split a 128-row tile with a long K chunk
 //    (128x64x32, or 128x128x64 for BF16) until every CU holds two (128x64) or
 //    one (128x128x64) workgroups;
+//  - medium outputs (autotuner comparisons on one MI355X): BF16 splits K
+//    over the large tile unless that leaves CUs idle with slices of at
+//    most 512, then over 128x128 tiles toward two workgroups per CU
+//    (1024x1024x4096: 213 TFLOP/s against 160 over the large tile; at
+//    1536^2x4096 the large tile with 1024-deep slices stays ahead); FP32
+//    runs the large tile with K split in two when that makes about one
+//    round of workgroups (2048^3: 112 against 108 for 128x64); FP64 takes
+//    the 128x128 tile when it fills the CUs, K split in two unless its
+//    tiles make whole rounds and K < 4096 (2048^3: 65 against 54;
+//    3072x2048x2048: 65 against 50 unsplit); FP32 128x64 tiles split K in
+//    two when the last round would be at most half full (3072^3: 119
+//    against 110);
 //  - otherwise 128x128 (BF16) / 128x64 tiles while they give every CU a
 //    workgroup, else 64x64; split K when the tiles give the CUs fewer than
 //    two workgroups each, aiming at three, with K slices at least 256 deep
@@ -142,7 +154,11 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
   if (split_mode(mode)) {
     const int64_t covered = (int64_t)lt * kDims[mode][cfg].bm * kDims[mode][cfg].bn;
     const int split = lt >= kCUs / 2 ? 1 : clamp_split(kCUs / lt, k, 512, kCUs);
-    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2) {
+    // K is split over the large tile while that gives half the CUs a
+    // workgroup, unless the slices end up at most 512 deep and the CUs are
+    // still not all busy (then 128x128 tiles below do better).
+    const bool thin = split > 1 && (int64_t)lt * split < kCUs && k / split <= 512;
+    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
       if (split == 1 && (lt < kCUs || (int64_t)m * n >= 3072 * ((int64_t)m + n)))
@@ -153,6 +169,8 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     const int rem = lt % kCUs;
     const bool even_out = mode == 0 && rem > 0 && rem <= kCUs / 2 && lt < 8 * kCUs && k >= 2048;
     return Choice{cfg, even_out ? 2 : 1};
+  } else if (mode == 0 && lt >= kCUs / 2 && 2 * lt <= kCUs + kCUs / 4 && k >= 2048) {
+    return Choice{cfg, 2};  // the large FP32 tile, K split in two, about one round
   }
   const int dc = deep_cfg(mode, m, n);
   const int dtiles = tiles_of(mode, dc, m, n);
@@ -160,11 +178,32 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     const int per_cu = (mode == 1 || dc == 2) ? 2 : 1;
     return Choice{dc, clamp_split(cdiv(per_cu * kCUs, dtiles), k, 512, kCUs)};
   }
+  if (mode == 1 && tiles_of(mode, 3, m, n) >= kCUs) {
+    // FP64 128x128 (copy after the MFMAs); K split in two unless the tiles
+    // make whole rounds of CUs and K is short.
+    const int t3 = tiles_of(mode, 3, m, n);
+    return Choice{3, (t3 % kCUs == 0 && k < 4096) ? 1 : 2};
+  }
   cfg = medium_cfg(mode);
-  if (tiles_of(mode, cfg, m, n) < kCUs) cfg = 1;  // 64x64
+  if (tiles_of(mode, cfg, m, n) < kCUs) {
+    // BF16: 128x128 with K split (a power of two) toward two workgroups per
+    // CU when that fills every CU with slices at least 512 deep; otherwise
+    // 64x64.
+    const int t0 = tiles_of(mode, cfg, m, n);
+    int want = 1;
+    while (want * 2 * t0 <= 2 * kCUs) want *= 2;
+    const int s0 = clamp_split(want, k, 512);
+    if (split_mode(mode) && (int64_t)t0 * s0 >= kCUs) return Choice{cfg, s0};
+    cfg = 1;  // 64x64
+  }
   const int tiles = tiles_of(mode, cfg, m, n);
   int split = 1;
-  if (tiles < 2 * kCUs && k >= 1024) split = clamp_split(cdiv(3 * kCUs, tiles), k, 256);
+  if (tiles < 2 * kCUs && k >= 1024) {
+    split = clamp_split(cdiv(3 * kCUs, tiles), k, 256);
+  } else if (mode == 0 && k >= 2048) {
+    const int rem = tiles % kCUs;  // FP32: even out a half-empty last round
+    if (rem > 0 && rem <= kCUs / 2 && tiles < 8 * kCUs) split = 2;
+  }
   return Choice{cfg, split};
 }
 
