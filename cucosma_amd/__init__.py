@@ -23,6 +23,7 @@ from .gemm import (  # noqa: F401
     gemm,
     normwise_error,
     fp64_product,
+    release_workspace,
     sgemm,
     sgemm_bf16x3,
     sgemm_bf16x4,

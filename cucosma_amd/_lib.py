@@ -108,6 +108,7 @@ def _load():
         "cosma_gemm_ex": (i, [C.POINTER(GemmDesc)]),
         "cosma_workspace_query": (i, [C.POINTER(GemmDesc), C.POINTER(C.c_size_t)]),
         "cosma_workspace_set": (i, [vp, C.c_size_t]),
+        "cosma_workspace_release": (i, [C.POINTER(C.c_size_t)]),
         "cosma_sgemm": (i, [f, vp, i, vp, i, f, vp, i]),
         "cosma_dgemm": (i, [d, vp, i, vp, i, d, vp, i]),
         "cosma_fixed_shape": (None, [C.POINTER(i), C.POINTER(i), C.POINTER(i)]),

@@ -224,13 +224,28 @@ size_t workspace_need(const cosma_gemm_desc* d, int cfg, int split, size_t* plan
 
 std::mutex g_mu;
 struct DevState {
-  void* ws = nullptr;
-  size_t ws_bytes = 0;
   double* stat_part = nullptr;  // kStatParts * 5 doubles + 5 doubles of output
 };
 std::vector<DevState> g_dev;
 void* g_user_ws = nullptr;
 size_t g_user_ws_bytes = 0;
+
+// Library-owned workspace: one buffer per (device, stream), so calls on
+// different streams never share partials or planes.  A buffer handed to a
+// call while its stream was capturing is referenced by the captured graph,
+// which may be replayed at any later time: it is never freed (growth retires
+// it to g_ws_retired).
+struct WsBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool captured = false;
+};
+std::map<std::pair<int, hipStream_t>, WsBuf> g_ws;
+std::vector<void*> g_ws_retired;
+
+// Serialises the residual-statistics calls: they share one partial buffer per
+// device and return only after their stream has finished with it.
+std::mutex g_stats_mu;
 
 DevState* dev_state(int* status) {
   int dev = 0;
@@ -250,8 +265,8 @@ bool capturing(hipStream_t s) {
 }
 
 // Resolve the split-K buffer: the descriptor's, the installed one, or the
-// library-owned one (allocated or grown only when the stream is not being
-// captured).
+// library-owned one of (current device, s), allocated or grown only when s
+// is not being captured.
 int resolve_workspace(const cosma_gemm_desc* d, size_t need, hipStream_t s, void** out) {
   if (d->workspace) {
     if (d->workspace_bytes < need) return COSMA_STATUS_INVALID_VALUE;
@@ -263,27 +278,34 @@ int resolve_workspace(const cosma_gemm_desc* d, size_t need, hipStream_t s, void
     *out = g_user_ws;
     return 0;
   }
-  int status = 0;
-  DevState* ds = dev_state(&status);
-  if (!ds) return status;
-  if (ds->ws_bytes < need) {
-    if (capturing(s)) return COSMA_STATUS_WORKSPACE;
-    if (ds->ws) {
-      hipError_t e = hipDeviceSynchronize();
-      if (e != hipSuccess) return (int)e;
-      e = hipFree(ds->ws);
-      ds->ws = nullptr;
-      ds->ws_bytes = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  const bool cap = capturing(s);
+  WsBuf& b = g_ws[{dev, s}];
+  if (b.bytes < need) {
+    if (cap) return COSMA_STATUS_WORKSPACE;
+    if (b.p) {
+      if (b.captured) {
+        g_ws_retired.push_back(b.p);
+      } else {
+        // only work queued on s uses this buffer
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return (int)e;
+        e = hipFree(b.p);
+      }
+      b = WsBuf{};
       if (e != hipSuccess) return (int)e;
     }
-    hipError_t e = hipMalloc(&ds->ws, need);
+    e = hipMalloc(&b.p, need);
     if (e != hipSuccess) {
-      ds->ws = nullptr;
+      b.p = nullptr;
       return (int)e;
     }
-    ds->ws_bytes = need;
+    b.bytes = need;
   }
-  *out = ds->ws;
+  if (cap) b.captured = true;
+  *out = b.p;
   return 0;
 }
 
@@ -827,6 +849,31 @@ int cosma_workspace_set(void* ptr, size_t bytes) {
   return 0;
 }
 
+int cosma_workspace_release(size_t* freed_bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (freed_bytes) *freed_bytes = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  bool synced = false;
+  for (auto it = g_ws.begin(); it != g_ws.end();) {
+    if (it->first.first != dev || it->second.captured) {
+      ++it;
+      continue;
+    }
+    if (!synced) {
+      e = hipDeviceSynchronize();
+      if (e != hipSuccess) return (int)e;
+      synced = true;
+    }
+    e = hipFree(it->second.p);
+    if (e != hipSuccess) return (int)e;
+    if (freed_bytes) *freed_bytes += it->second.bytes;
+    it = g_ws.erase(it);
+  }
+  return 0;
+}
+
 int cosma_sgemm(float alpha, const float* A, int lda, const float* B, int ldb, float beta,
                 float* C, int ldc) {
   return fixed_gemm<COSMA_MODE_FP32, COSMA_M, COSMA_N, COSMA_K>(alpha, A, lda, B, ldb, beta, C, ldc);
@@ -923,6 +970,7 @@ int cosma_error_stats_f32(int rows, int cols, const float* C, int64_t ldc, const
                           int64_t ldr, double* out5, void* stream) {
   if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols)
     return COSMA_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_stats_mu);
   double *part, *dout;
   int st = stats_scratch(&part, &dout);
   if (st) return st;
@@ -933,6 +981,7 @@ int cosma_error_stats_f64(int rows, int cols, const double* C, int64_t ldc, cons
                           int64_t ldr, double* out5, void* stream) {
   if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols)
     return COSMA_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_stats_mu);
   double *part, *dout;
   int st = stats_scratch(&part, &dout);
   if (st) return st;
@@ -942,6 +991,7 @@ int cosma_error_stats_f64(int rows, int cols, const double* C, int64_t ldc, cons
 int cosma_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* out5, void* stream) {
   if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && !X) || ldx < cols)
     return COSMA_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_stats_mu);
   double *part, *dout;
   int st = stats_scratch(&part, &dout);
   if (st) return st;
@@ -951,6 +1001,7 @@ int cosma_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* out
 int cosma_sumsq_f64(int rows, int cols, const double* X, int64_t ldx, double* out5, void* stream) {
   if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && !X) || ldx < cols)
     return COSMA_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_stats_mu);
   double *part, *dout;
   int st = stats_scratch(&part, &dout);
   if (st) return st;

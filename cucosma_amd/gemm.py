@@ -124,6 +124,15 @@ def workspace_bytes(m: int, n: int, k: int, *, mode: str = "fp32", split_k: int 
     return int(out.value)
 
 
+def release_workspace() -> int:
+    """Free the current device's library-owned workspace buffers (one per
+    stream that ran a split-K or pre-split call); returns the bytes freed.
+    Synchronises the device."""
+    out = C.c_size_t()
+    check(lib.cosma_workspace_release(C.byref(out)), "cosma_workspace_release")
+    return int(out.value)
+
+
 def sgemm(A, B, C_out=None, alpha=1.0, beta=0.0, **kw):
     return gemm(A, B, C_out, alpha=alpha, beta=beta, mode="fp32", **kw)
 

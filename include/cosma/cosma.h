@@ -83,7 +83,7 @@ typedef struct cosma_gemm_desc {
     int split_k;            /* <= 0: library heuristic; else 1..4096 */
     int tile_cfg;           /* < 0: library heuristic; else index < cosma_num_tile_configs(mode) */
     int group_m;            /* <= 0: default (8) tile rows per grouped-order group */
-    void *workspace;        /* split-K partials; NULL: library-owned buffer */
+    void *workspace;        /* split-K partials and planes; NULL: installed or library-owned buffer */
     size_t workspace_bytes; /* size of workspace */
     void *stream;           /* hipStream_t; NULL: default stream */
     int epilogue;           /* COSMA_EPILOGUE_*: activation applied to the final C */
@@ -101,16 +101,32 @@ void cosma_gemm_desc_init(cosma_gemm_desc *d);
  * (Kernel/cuda/src/cuCOSMAV100.cuh:2774-2848) for every mode and layout. */
 int cosma_gemm_ex(const cosma_gemm_desc *d);
 
-/* Bytes of split-K workspace the call *d needs (0 when split_k resolves to 1). */
+/* Bytes of workspace the call *d needs: split-K partials, then the planes of
+ * operands split inside the call (0 when neither applies). */
 int cosma_workspace_query(const cosma_gemm_desc *d, size_t *bytes);
 
-/* Install a caller-owned device buffer used for split-K partials by calls
- * whose descriptor brings no workspace.  NULL/0 reverts to the library-owned
- * buffer.  The reference allocates and frees the partial buffer inside every
- * call (cuCOSMAV100.cuh:2826-2838); here it is allocated at most once, never
- * while the stream is capturing.  Calls that share a workspace must be
- * ordered on one stream. */
+/* Install a caller-owned device buffer used for split-K partials (and the
+ * planes of operands split inside the call) by calls whose descriptor brings
+ * no workspace.  NULL/0 reverts to the library-owned buffers.  Calls that
+ * share an installed workspace must be ordered on one stream.
+ *
+ * The reference allocates and frees the partial buffer inside every call
+ * (cuCOSMAV100.cuh:2826-2838).  Here, without a caller or installed buffer,
+ * each (device, stream) pair owns one library buffer, allocated on first need
+ * and grown only when the stream is not capturing (COSMA_STATUS_WORKSPACE
+ * then): calls on different streams never share one.  Growing waits for the
+ * stream.  A buffer given to a call while its stream was capturing is held
+ * until the process ends, because the captured graph may be replayed later;
+ * replay such a graph ordered with other calls on the stream it was captured
+ * on. */
 int cosma_workspace_set(void *ptr, size_t bytes);
+
+/* Free the current device's library-owned workspace buffers, after
+ * synchronising the device (the memory cudaFree returns at the end of every
+ * reference call, cuCOSMAV100.cuh:2838).  Buffers a captured graph uses are
+ * kept.  *freed_bytes (may be NULL) receives the bytes freed.  Not while any
+ * stream is capturing. */
+int cosma_workspace_release(size_t *freed_bytes);
 
 /* Pre-split operands (BF16X3 / BF16X4).  The split modes can split each FP32
  * operand once into bf16 hi/lo "planes" -- rows of 2 * Kp bf16, Kp = K

@@ -378,6 +378,41 @@ def test_caller_workspace(cc, dev):
     assert np.array_equal(c1, c2)
 
 
+def test_library_workspace_per_stream(cc, dev):
+    """Calls on two streams without a caller workspace use separate library
+    buffers: pre-split planes and split-K partials queued on both streams at
+    once give the results of the same calls run one after another, bitwise.
+    Releasing frees the buffers; later calls allocate again."""
+    g = torch.Generator(device="cpu").manual_seed(9)
+    M = N = 2048
+    K = 4096
+    ops = [((torch.rand(M, K, generator=g) - 0.5).to(dev), (torch.rand(K, N, generator=g) - 0.5).to(dev))
+           for _ in range(2)]
+    kws = [dict(mode="bf16x3", tile_cfg=8, split_k=1), dict(mode="fp32", tile_cfg=1, split_k=8)]
+    need = max(cc.workspace_bytes(M, N, K, **kw) for kw in kws)
+    assert need == 8 * M * N * 4
+    refs = [[cc.gemm(A, B, **kw) for kw in kws] for A, B in ops]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [[torch.empty(M, N, device=dev) for _ in kws] for _ in ops]
+    for s in streams:
+        s.wait_stream(torch.cuda.current_stream())
+    for _ in range(4):
+        for j, kw in enumerate(kws):
+            for (A, B), s, out in zip(ops, streams, outs):
+                cc.gemm(A, B, out[j], stream=s, **kw)
+    torch.cuda.synchronize()
+    for i in range(2):
+        for j in range(2):
+            assert torch.equal(outs[i][j], refs[i][j]), (i, kws[j])
+    # the current stream's buffer and one per side stream
+    assert cc.release_workspace() >= 3 * need
+    assert cc.release_workspace() == 0
+    A, B = ops[0]
+    assert torch.equal(cc.gemm(A, B, **kws[0]), refs[0][0])
+    torch.cuda.synchronize()
+
+
 def test_fp64_device_reference_matches_host_oracle(cc, dev):
     """The GPU FP64 product used for large-size error reports agrees with the
     host FP64 oracle."""
