@@ -101,6 +101,21 @@ struct Traits<kBF16x3> {
 template <>
 struct Traits<kBF16x4> : Traits<kBF16x3> {};
 
+// Global-memory (address space 1) views of operand pointers.  A pointer the
+// kernel reads from memory -- the grouped launch's problem table -- is a
+// generic pointer to the compiler, which then emits flat loads and stores.
+// Flat accesses count against lgkmcnt as well as vmcnt, so every wait for an
+// LDS fragment read would also wait for all operand loads in flight and the
+// prefetch would stop overlapping the MFMAs.  Loads and stores through these
+// views are global instructions whatever the pointer's origin (kernel
+// arguments are inferred global already).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+
 template <int MODE>
 struct GemmParams {
   using G = typename Traits<MODE>::G;
@@ -241,8 +256,8 @@ struct TileLoader {
   // A whole 16-byte vector when it is in range and aligned; otherwise guarded
   // element loads (EDGE only: the unchecked instantiation is launched only
   // for aligned, tile-divisible problems).
-  __device__ __forceinline__ static GV load_vec(const G* p, bool row_ok, int e_valid, bool vec_ok) {
-    if (!EDGE || (vec_ok && row_ok && e_valid >= GVEC)) return *reinterpret_cast<const GV*>(p);
+  __device__ __forceinline__ static GV load_vec(gptr<const G> p, bool row_ok, int e_valid, bool vec_ok) {
+    if (!EDGE || (vec_ok && row_ok && e_valid >= GVEC)) return *reinterpret_cast<gptr<const GV>>(p);
     GV out;
 #pragma unroll
     for (int e = 0; e < GVEC; ++e) out[e] = (row_ok && e < e_valid) ? p[e] : G(0);
@@ -255,7 +270,7 @@ struct TileLoader {
   static constexpr int VPR = BK / GVEC;  // KMAJ: vectors per row; otherwise k blocks per tile
   static constexpr int UNIT_STEP = KMAJ ? NT / VPR : (NT / VPR) * GVEC;
   static_assert(NT % VPR == 0, "unit stride must be uniform across the workgroup");
-  const G* p0;
+  gptr<const G> p0;
   int r0_;  // KMAJ: row of the first unit; otherwise its first column
   int k0_;  // first k of every unit of this thread
   int64_t ld_;
@@ -269,11 +284,11 @@ struct TileLoader {
     if constexpr (KMAJ) {
       r0_ = r0 + a;
       k0_ = k0 + b * GVEC;
-      p0 = base + (int64_t)r0_ * ld + k0_;
+      p0 = to_global(base) + (int64_t)r0_ * ld + k0_;
     } else {
       r0_ = r0 + a * GVEC;
       k0_ = k0 + b * GVEC;
-      p0 = base + (int64_t)k0_ * ld + r0_;
+      p0 = to_global(base) + (int64_t)k0_ * ld + r0_;
     }
   }
 
@@ -286,10 +301,10 @@ struct TileLoader {
       const int rr = r0_ + i * UNIT_STEP;
       const int gk = k0_ + kofs;
       if constexpr (KMAJ) {
-        const G* p = p0 + (int64_t)(i * UNIT_STEP) * ld_ + kofs;
+        gptr<const G> p = p0 + (int64_t)(i * UNIT_STEP) * ld_ + kofs;
         v[i] = load_vec(p, rr < rmax, kend - gk, vec_ok);
       } else {
-        const G* p = p0 + (int64_t)kofs * ld_ + i * UNIT_STEP;
+        gptr<const G> p = p0 + (int64_t)kofs * ld_ + i * UNIT_STEP;
 #pragma unroll
         for (int u = 0; u < GVEC; ++u)
           v[i * GVEC + u] = load_vec(p + (int64_t)u * ld_, gk + u < kend, rmax - rr, vec_ok);
@@ -572,7 +587,8 @@ __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int r
   const int lcol = col0 + (lane & 15);
   const bool partial = p.split > 1;
   const int64_t ld = partial ? (int64_t)p.N : p.ldc;
-  E* base = partial ? p.ws + ((int64_t)z * p.M + lrow) * p.N + lcol : p.C + (int64_t)lrow * p.ldc + lcol;
+  gptr<E> base = to_global(partial ? p.ws + ((int64_t)z * p.M + lrow) * p.N + lcol
+                                    : p.C + (int64_t)lrow * p.ldc + lcol);
   const E scale = partial ? E(1) : p.alpha;
   const bool read_c = !partial && p.beta != E(0);
   const int act = partial ? kActNone : p.act;
@@ -582,7 +598,7 @@ __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int r
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int ro = i * 16 + (MODE == kF64 ? 4 * r : r);  // row offset of element r from lrow
-      E* q = base + (int64_t)ro * ld;
+      gptr<E> q = base + (int64_t)ro * ld;
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         if (EDGE && (lrow + ro >= p.M || lcol + j * 16 >= p.N)) continue;
