@@ -19,7 +19,8 @@ Configurations:
            variants, cuCOSMAV100.cuh:2466-2476 and sigmoid4.cuh:13-27)
   dgemm    32768x1024x1024 and 1024x65536x1024, normal(0,1) seed 2 with 1% of
            entries x1e6, C normal(0,1), alpha = beta = 1
-  grouped  512 problems, M/N/K log-uniform in [32, 2048] rounded to 32, seed 3:
+  grouped  512 problems, M/N/K log-uniform in [32, 2048] rounded to 32, seed 3,
+           fp32, bf16x3, bf16x4 and fp64 (FP64 storage for fp64):
            one hipGraph replay of the library's default grouped plan vs one
            launch per problem, and every assignment of 64x64 / 128x128 /
            large tiles to three output-size classes for comparison
@@ -213,15 +214,19 @@ def grouped_sizes(count=512, seed=3, lo=32, hi=2048):
 def cfg_grouped(cc, torch, args, emit):
     sizes = grouped_sizes()
     flops = sum(2.0 * m * n * k for m, n, k in sizes)
-    for mode in ("fp32", "bf16x3"):
+    for mode in ("fp32", "bf16x3", "bf16x4", "fp64"):
+        dt = torch.float64 if mode == "fp64" else torch.float32
         g = torch.Generator(device="cuda").manual_seed(3)
-        As = [torch.rand((m, k), device="cuda", generator=g) * 2 - 1 for m, n, k in sizes]
-        Bs = [torch.rand((k, n), device="cuda", generator=g) * 2 - 1 for m, n, k in sizes]
-        Cs = [torch.empty((m, n), device="cuda") for m, n, k in sizes]
+        As = [torch.rand((m, k), device="cuda", generator=g, dtype=dt) * 2 - 1 for m, n, k in sizes]
+        Bs = [torch.rand((k, n), device="cuda", generator=g, dtype=dt) * 2 - 1 for m, n, k in sizes]
+        Cs = [torch.empty((m, n), device="cuda", dtype=dt) for m, n, k in sizes]
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
 
         def time_plan(plan, reps):
-            plan.run_graph()
+            # warm-up replays first, so that every plan (the first one timed
+            # included) runs at the clocks of a busy GPU
+            for _ in range(20):
+                plan.run_graph()
             torch.cuda.synchronize()
             ev[0].record()
             for _ in range(reps):
@@ -240,7 +245,7 @@ def cfg_grouped(cc, torch, args, emit):
         edges = (256, 1024)
         bucket = [0 if max(m, n) < edges[0] else (1 if max(m, n) < edges[1] else 2)
                   for m, n, k in sizes]
-        large = {"fp32": 7, "bf16x3": 6}[mode]
+        large = {"fp32": 7, "bf16x3": 6, "bf16x4": 3, "fp64": 7}[mode]
         cand = (1, 0, large)
         tried = []
         for c0 in () if args.no_search else cand:
