@@ -17,8 +17,22 @@ __global__ void __launch_bounds__(256)
   const int64_t total = (int64_t)M * N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    // Loads of kUnroll slices are issued before their adds, so each thread
+    // keeps that many independent loads in flight (the sum is latency-bound
+    // otherwise: one element per thread, S dependent steps); the adds still
+    // run one slice after another from zero.
+    constexpr int kUnroll = sizeof(E) == 4 ? 32 : 16;
+    const E* p = ws + e;
     E v = E(0);
-    for (int s = 0; s < S; ++s) v += ws[(int64_t)s * total + e];
+    int s = 0;
+    for (; s + kUnroll <= S; s += kUnroll) {
+      E t[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) t[u] = p[(int64_t)(s + u) * total];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) v += t[u];
+    }
+    for (; s < S; ++s) v += p[(int64_t)s * total];
     const int row = (int)(e / N);
     const int col = (int)(e - (int64_t)row * N);
     E* cp = C + (int64_t)row * ldc + col;

@@ -156,6 +156,25 @@ def test_splitk_deterministic_and_consistent(cc, dev):
             _bound_check(first, ref, A, B, 2.0, 2.0**-24, extra)
 
 
+def test_splitk_reduce_is_ascending_slice_sum(cc, dev):
+    """A split-K result equals, bitwise, the sum in ascending slice order of
+    single launches over the K slices (same tile, the slice boundaries on BK
+    multiples), i.e. the reduction adds the partials one slice after another
+    from zero, like the reference's reduce loop."""
+    M, N, K, S = 96, 160, 8192, 16
+    ks = K // S
+    g = torch.Generator(device="cpu").manual_seed(11)
+    for mode, dt in (("fp32", torch.float32), ("fp64", torch.float64), ("bf16x3", torch.float32)):
+        A = (torch.rand(M, K, generator=g, dtype=dt) * 2 - 1).to(dev)
+        B = (torch.rand(K, N, generator=g, dtype=dt) * 2 - 1).to(dev)
+        got = cc.gemm(A, B, mode=mode, split_k=S, tile_cfg=1)
+        acc = torch.zeros(M, N, dtype=dt, device=dev)
+        for z in range(S):
+            part = cc.gemm(A[:, z * ks:(z + 1) * ks], B[z * ks:(z + 1) * ks], mode=mode, split_k=1, tile_cfg=1)
+            acc = acc + part
+        assert torch.equal(got, acc), mode
+
+
 def test_fp64_vs_long_double_with_outliers(cc, dev):
     A, B, C0 = oracle.workload_dgemm(200, 150, 300, seed=2)
     ref = oracle.gemm_ld(A, B, C0, 1.0, 1.0)
