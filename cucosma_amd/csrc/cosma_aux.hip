@@ -10,29 +10,38 @@ namespace cosma {
 // depend on launch order or timing (Kernel/cuda/src/cosmaSplitKReduce.cuh:20-42
 // sums in the same order).  beta == 0 does not read C.  The activation runs
 // here, on the reduced value, not on each slice's partial.
-template <typename E>
+//
+// One element per thread is a chain of S loads.  With U > 1 a thread issues
+// the loads of U slices before adding them (still one slice after another,
+// from zero), so U loads per thread are in flight; the last, partial batch
+// loads and adds only the slices that exist.  The U per-slice offsets cost
+// scalar set-up in every wave, so the host picks U by S (U = 1 is the plain
+// loop, for few slices).
+template <typename E, int U>
 __global__ void __launch_bounds__(256)
     splitk_reduce_kernel(const E* __restrict__ ws, E* __restrict__ C, int64_t ldc, int M, int N,
                          int S, E alpha, E beta, int act) {
   const int64_t total = (int64_t)M * N;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-    // Loads of kUnroll slices are issued before their adds, so each thread
-    // keeps that many independent loads in flight (the sum is latency-bound
-    // otherwise: one element per thread, S dependent steps); the adds still
-    // run one slice after another from zero.
-    constexpr int kUnroll = sizeof(E) == 4 ? 32 : 16;
     const E* p = ws + e;
     E v = E(0);
     int s = 0;
-    for (; s + kUnroll <= S; s += kUnroll) {
-      E t[kUnroll];
+    for (; s + U <= S; s += U) {
+      E t[U];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) t[u] = p[(int64_t)(s + u) * total];
+      for (int u = 0; u < U; ++u) t[u] = p[(int64_t)(s + u) * total];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) v += t[u];
+      for (int u = 0; u < U; ++u) v += t[u];
     }
-    for (; s < S; ++s) v += p[(int64_t)s * total];
+    if (U > 1 && s < S) {
+      E t[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) t[u] = s + u < S ? p[(int64_t)(s + u) * total] : E(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (s + u < S) v += t[u];
+    }
     const int row = (int)(e / N);
     const int col = (int)(e - (int64_t)row * N);
     E* cp = C + (int64_t)row * ldc + col;
@@ -245,17 +254,32 @@ hipError_t run_scale_f64(double* C, int64_t ldc, int M, int N, double beta, int 
   return hipGetLastError();
 }
 
+// Loads in flight per thread: one (the plain loop) when the slices are few
+// or the elements alone give every CU its 32 resident waves (524288 elements
+// and up: 1024^2 with 16 slices ran 14.7 us against 15.8 with U = 8; 1024^2
+// with 3 slices 5.7 us against 11.4 with U = 32); else 8, or 32 (FP32) / 16
+// (FP64) from 64 slices (256x256 with 256 slices: 12 us against 75 with one
+// load at a time).
+template <typename E>
+static hipError_t launch_reduce(const E* ws, E* C, int64_t ldc, int M, int N, int S, E alpha, E beta,
+                                int act, hipStream_t s) {
+  const dim3 grid(grid_for((int64_t)M * N, 256, 8192)), block(256);
+  if (S < 8 || (int64_t)M * N >= (int64_t)256 * 32 * 64)
+    hipLaunchKernelGGL((splitk_reduce_kernel<E, 1>), grid, block, 0, s, ws, C, ldc, M, N, S, alpha, beta, act);
+  else if (S < 64)
+    hipLaunchKernelGGL((splitk_reduce_kernel<E, 8>), grid, block, 0, s, ws, C, ldc, M, N, S, alpha, beta, act);
+  else
+    hipLaunchKernelGGL((splitk_reduce_kernel<E, sizeof(E) == 4 ? 32 : 16>), grid, block, 0, s, ws, C, ldc, M,
+                       N, S, alpha, beta, act);
+  return hipGetLastError();
+}
 hipError_t run_reduce_f32(const float* ws, float* C, int64_t ldc, int M, int N, int S, float alpha,
                           float beta, int act, hipStream_t s) {
-  hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(grid_for((int64_t)M * N, 256, 8192)),
-                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta, act);
-  return hipGetLastError();
+  return launch_reduce<float>(ws, C, ldc, M, N, S, alpha, beta, act, s);
 }
 hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N, int S,
                           double alpha, double beta, int act, hipStream_t s) {
-  hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(grid_for((int64_t)M * N, 256, 8192)),
-                     dim3(256), 0, s, ws, C, ldc, M, N, S, alpha, beta, act);
-  return hipGetLastError();
+  return launch_reduce<double>(ws, C, ldc, M, N, S, alpha, beta, act, s);
 }
 
 // Cache flush: overwrite a buffer larger than L2 + Infinity Cache.

@@ -161,18 +161,33 @@ def test_splitk_reduce_is_ascending_slice_sum(cc, dev):
     single launches over the K slices (same tile, the slice boundaries on BK
     multiples), i.e. the reduction adds the partials one slice after another
     from zero, like the reference's reduce loop."""
-    M, N, K, S = 96, 160, 8192, 16
-    ks = K // S
+    M, N, K = 96, 160, 8192
     g = torch.Generator(device="cpu").manual_seed(11)
-    for mode, dt in (("fp32", torch.float32), ("fp64", torch.float64), ("bf16x3", torch.float32)):
-        A = (torch.rand(M, K, generator=g, dtype=dt) * 2 - 1).to(dev)
-        B = (torch.rand(K, N, generator=g, dtype=dt) * 2 - 1).to(dev)
+    # the reduction's load batches for this output size: 3 slices one load at a
+    # time, 16 and 20 in 8-wide batches (20: a partial last batch), 64 and 67
+    # in 32-wide (FP32) / 16-wide (FP64) batches (67: a partial last batch)
+    cases = (("fp32", torch.float32, 16), ("fp64", torch.float64, 20), ("bf16x3", torch.float32, 16),
+             ("fp32", torch.float32, 64), ("fp64", torch.float64, 64), ("fp32", torch.float32, 3))
+    for mode, dt, S in cases:
+        ks = K // S // 32 * 32
+        Kc = ks * S
+        A = (torch.rand(M, Kc, generator=g, dtype=dt) * 2 - 1).to(dev)
+        B = (torch.rand(Kc, N, generator=g, dtype=dt) * 2 - 1).to(dev)
         got = cc.gemm(A, B, mode=mode, split_k=S, tile_cfg=1)
         acc = torch.zeros(M, N, dtype=dt, device=dev)
         for z in range(S):
             part = cc.gemm(A[:, z * ks:(z + 1) * ks], B[z * ks:(z + 1) * ks], mode=mode, split_k=1, tile_cfg=1)
             acc = acc + part
-        assert torch.equal(got, acc), mode
+        assert torch.equal(got, acc), (mode, S)
+    # 67 slices of 96
+    S, ks = 67, 96
+    A = (torch.rand(M, S * ks, generator=g) * 2 - 1).to(dev)
+    B = (torch.rand(S * ks, N, generator=g) * 2 - 1).to(dev)
+    got = cc.gemm(A, B, mode="fp32", split_k=S, tile_cfg=1)
+    acc = torch.zeros(M, N, device=dev)
+    for z in range(S):
+        acc = acc + cc.gemm(A[:, z * ks:(z + 1) * ks], B[z * ks:(z + 1) * ks], mode="fp32", split_k=1, tile_cfg=1)
+    assert torch.equal(got, acc)
 
 
 def test_fp64_vs_long_double_with_outliers(cc, dev):
