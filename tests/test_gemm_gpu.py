@@ -447,6 +447,58 @@ def test_library_workspace_per_stream(cc, dev):
     torch.cuda.synchronize()
 
 
+def test_gemm_calls_in_a_graph(cc, dev):
+    """A split-K call (partials in the library workspace) and a pre-split call
+    (planes in the workspace), captured in one hipGraph on a stream whose
+    workspace was sized by an eager call, replay bitwise equal to the eager
+    results, with eager work between replays and after releasing the
+    library's workspaces (the captured buffer is kept).  A call that needs a
+    larger workspace while the stream is capturing fails before launching."""
+    g = torch.Generator(device="cpu").manual_seed(12)
+    A1 = (torch.rand(128, 4096, generator=g) * 2 - 1).to(dev)
+    B1 = (torch.rand(4096, 192, generator=g) * 2 - 1).to(dev)
+    A2 = (torch.rand(1024, 512, generator=g) * 2 - 1).to(dev)
+    B2 = (torch.rand(512, 768, generator=g) * 2 - 1).to(dev)
+    C1 = torch.empty(128, 192, device=dev)
+    C2 = torch.empty(1024, 768, device=dev)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+
+    def calls():
+        cc.gemm(A1, B1, C1, mode="fp32", split_k=8, tile_cfg=1)
+        cc.gemm(A2, B2, C2, mode="bf16x3", tile_cfg=8)
+
+    with torch.cuda.stream(s):
+        calls()
+    torch.cuda.synchronize()
+    ref1, ref2 = C1.clone(), C2.clone()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        calls()
+    torch.cuda.synchronize()
+    for rep in range(4):
+        C1.zero_()
+        C2.zero_()
+        torch.cuda.synchronize()
+        if rep == 2:
+            cc.release_workspace()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C1, ref1) and torch.equal(C2, ref2), rep
+    A3 = (torch.rand(128, 16384, generator=g) * 2 - 1).to(dev)
+    B3 = (torch.rand(16384, 192, generator=g) * 2 - 1).to(dev)
+    C3 = torch.empty(128, 192, device=dev)
+    assert cc.workspace_bytes(128, 192, 16384, split_k=64, tile_cfg=1) > cc.workspace_bytes(
+        1024, 768, 512, mode="bf16x3", tile_cfg=8)
+    graph2 = torch.cuda.CUDAGraph()
+    with pytest.raises(cc.CosmaError) as err:
+        with torch.cuda.graph(graph2, stream=s):
+            cc.gemm(A3, B3, C3, mode="fp32", split_k=64, tile_cfg=1)
+    assert err.value.status == -4
+    torch.cuda.synchronize()
+    del graph, graph2
+
+
 def test_fp64_device_reference_matches_host_oracle(cc, dev):
     """The GPU FP64 product used for large-size error reports agrees with the
     host FP64 oracle."""
