@@ -533,7 +533,7 @@ struct cosma_grouped_plan_s {
     int tiles;
     int blocks;
     void* table;        // device GroupEntry array, then the schedule
-    const int* sched;   // blocks + 1 offsets, then tile ids (inside table's allocation)
+    const int* sched;   // blocks + 1 offsets, then (problem, tile) pairs (inside table's allocation)
   };
   std::vector<Bucket> buckets;
   double flops = 0.0;
@@ -581,7 +581,7 @@ int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*
     e.alpha = q->alpha;
     e.beta = q->beta;
     tiles += (int64_t)cdiv(q->m, td.bm) * e.tiles_n;
-    if (tiles > 0x7fffffff) return COSMA_STATUS_INVALID_VALUE;
+    if (tiles > 0x3fffffff) return COSMA_STATUS_INVALID_VALUE;  // two schedule ints per tile
     table.push_back(e);
     cost.push_back(cdiv(q->k, td.bk) + kTileOverheadSteps);
   }
@@ -595,7 +595,7 @@ int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*
   out->blocks = B;
 
   // Longest-processing-time-first schedule (see above).
-  std::vector<std::vector<int>> lists(B);
+  std::vector<std::vector<std::pair<int, int>>> lists(B);  // (problem, tile within it)
   using Slot = std::pair<int64_t, int>;  // (load, logical workgroup)
   std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
   for (int l = 0; l < B; ++l) heap.push(Slot{0, l});
@@ -605,18 +605,22 @@ int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*
     for (int64_t t = first; t < last; ++t) {
       Slot s = heap.top();
       heap.pop();
-      lists[s.second].push_back((int)t);
+      lists[s.second].push_back({(int)i, (int)(t - first)});
       heap.push(Slot{s.first + cost[i], s.second});
     }
   }
   const int per_xcd = B % 8 == 0 ? B / 8 : 0;
-  std::vector<int> sched((size_t)B + 1 + (size_t)tiles);
+  std::vector<int> sched((size_t)B + 1 + 2 * (size_t)tiles);
   std::vector<int> of_phys(B);  // logical workgroup run by physical id
   for (int l = 0; l < B; ++l) of_phys[per_xcd ? (l % per_xcd) * 8 + l / per_xcd : l] = l;
   int pos = 0;
   for (int w = 0; w < B; ++w) {
     sched[w] = pos;
-    for (int t : lists[of_phys[w]]) sched[(size_t)B + 1 + pos++] = t;
+    for (const auto& pt : lists[of_phys[w]]) {
+      sched[(size_t)B + 1 + 2 * (size_t)pos] = pt.first;
+      sched[(size_t)B + 1 + 2 * (size_t)pos + 1] = pt.second;
+      ++pos;
+    }
   }
   sched[B] = pos;
 

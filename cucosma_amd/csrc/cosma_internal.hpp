@@ -84,7 +84,8 @@ hipError_t run_gemm_bf16x4(const Problem& p);
 
 // Grouped persistent launch of `blocks` workgroups over `count` table
 // entries (device memory) of tile config cfg; sched (device memory) holds
-// blocks + 1 offsets and then the tile ids of each workgroup.
+// blocks + 1 offsets and then, for each workgroup's tiles in order, the pair
+// (table index of the problem, tile index within the problem).
 hipError_t run_grouped(int mode, int cfg, const void* table, int count, const int* sched, int blocks,
                        hipStream_t s);
 // Workgroups of the grouped kernel of (mode, cfg) that fit on one CU.

@@ -767,12 +767,13 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
 
 // ---------------------------------------------------------------------------
 // Grouped GEMM: many independent row-major problems in one persistent launch.
-// The table lists every problem with the index of its first tile in a global
-// tile numbering (tile_start, ascending).  sched holds the host-computed
-// schedule: sched[w] .. sched[w + 1] index the tiles of workgroup w in
-// sched[gridDim.x + 1 + i] (a longest-first assignment that balances the
-// K-steps per workgroup); each tile is located by binary search in the table
-// and computed with the same gemm_tile as the single-problem kernel (split 1,
+// The table lists every problem.  sched holds the host-computed schedule:
+// sched[w] .. sched[w + 1] index the tiles of workgroup w, each a pair
+// (problem, tile within the problem) at sched[gridDim.x + 1 + 2i] and the
+// next int (a longest-first assignment that balances the K-steps per
+// workgroup; the pair costs one table load per tile, where a search of the
+// table for a global tile index took a chain of dependent loads).  Each tile
+// is computed with the same gemm_tile as the single-problem kernel (split 1,
 // bounds-checked edges), so the schedule never changes a result.
 // ---------------------------------------------------------------------------
 template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, int OCC>
@@ -781,17 +782,14 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
                    const int* __restrict__ sched) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   using E = typename Traits<MODE>::E;
+  (void)count;  // table entries; the host writes only indices below it into sched
   __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
   const int* order = sched + gridDim.x + 1;
   const int end = sched[blockIdx.x + 1];
   for (int i = sched[blockIdx.x]; i < end; ++i) {
-    const int t = order[i];
-    int lo = 0, hi = count - 1;  // last entry with tile_start <= t
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (table[mid].tile_start <= t) lo = mid; else hi = mid - 1;
-    }
-    const GroupEntry<typename Traits<MODE>::G>& e = table[lo];
+    const int prob = order[2 * (int64_t)i];
+    const int local = order[2 * (int64_t)i + 1];
+    const GroupEntry<typename Traits<MODE>::G>& e = table[prob];
     GemmParams<MODE> p;
     p.A = e.A;
     p.B = e.B;
@@ -813,7 +811,6 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
     p.vec_b = ((uintptr_t)e.B % 16 == 0) && (e.ldb % Traits<MODE>::GVEC == 0);
     p.alpha = (E)e.alpha;
     p.beta = (E)e.beta;
-    const int local = t - e.tile_start;
     const int tm = local / e.tiles_n;
     const int tn = local - tm * e.tiles_n;
     gemm_tile<MODE, BM, BN, BK, WM, WN, PIPE, false, false, true>(p, tm, tn, 0, lds);
