@@ -577,7 +577,11 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
 // whether to read C are uniform and chosen once; one loop serves all cases
 // (separate loops per case made the compiler keep extra copies of the
 // accumulators, raising the kernel's register count).  Every address is a
-// per-lane base pointer plus a uniform (16 i + row) * ld offset.
+// per-lane base pointer plus a uniform (16 i + row) * ld offset.  With beta
+// != 0 the C values of one sub-tile row (4 x TN per lane) are loaded together
+// before any of them is stored: a load cannot move above an earlier store
+// that may alias it (ld is a run-time value), so loading each element just
+// before its own store paid one memory latency per element.
 template <int MODE, bool EDGE, int TM, int TN>
 __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int row0, int col0,
                                          int lane, const typename Traits<MODE>::Acc (&acc)[TM][TN]) {
@@ -594,20 +598,32 @@ __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int r
   const int act = partial ? kActNone : p.act;
   const E beta = p.beta;
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < TM; ++i) {
+    E cv[4][TN];
+    if (read_c) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ro = i * 16 + (MODE == kF64 ? 4 * r : r);  // row offset of element r from lrow
+        gptr<E> q = base + (int64_t)ro * ld;
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          cv[r][j] = (!EDGE || (lrow + ro < p.M && lcol + j * 16 < p.N)) ? q[j * 16] : E(0);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int ro = i * 16 + (MODE == kF64 ? 4 * r : r);  // row offset of element r from lrow
+      const int ro = i * 16 + (MODE == kF64 ? 4 * r : r);
       gptr<E> q = base + (int64_t)ro * ld;
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         if (EDGE && (lrow + ro >= p.M || lcol + j * 16 >= p.N)) continue;
         E out = scale * acc[i][j][r];
-        if (read_c) out = fma_e(beta, q[j * 16], out);
+        if (read_c) out = fma_e(beta, cv[r][j], out);
         if (act != kActNone) out = activate(out, act);
         q[j * 16] = out;
       }
     }
+  }
 }
 
 // One output tile (tm, tn) of split z: the K-slice main loop and the
