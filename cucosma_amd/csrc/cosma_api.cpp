@@ -200,6 +200,23 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
   int split = 1;
   if (tiles < 2 * kCUs && k >= 1024) {
     split = clamp_split(cdiv(3 * kCUs, tiles), k, 256);
+    if (mode == 1) {
+      // FP64 fits two 128x64 or four 64x64 workgroups per CU, not three: take
+      // the smallest split whose last round of workgroups is at least three
+      // quarters full (cache flushed, beta 0.5: 1024x2048x2048 185 us at
+      // split 2 against 239 at 3, 2048x1024x1024 108 against 139, 1792^3 287
+      // at split 1 against 307 at 2; the other shapes tried keep their split).
+      const int64_t slots = (int64_t)(cfg == 1 ? 4 : 2) * kCUs;
+      const int smax = clamp_split(64, k, 256);
+      for (int s = 1; s <= smax; ++s) {
+        const int64_t w = (int64_t)tiles * s;
+        const int64_t rounds = (w + slots - 1) / slots;
+        if (4 * w >= 3 * rounds * slots) {
+          split = s;
+          break;
+        }
+      }
+    }
   } else if (mode == 0 && k >= 2048) {
     const int rem = tiles % kCUs;  // FP32: even out a half-empty last round
     if (rem > 0 && rem <= kCUs / 2 && tiles < 8 * kCUs) split = 2;
