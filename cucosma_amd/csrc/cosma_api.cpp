@@ -168,7 +168,13 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
   } else if (lt >= 2 * kCUs) {
     const int rem = lt % kCUs;
     const bool even_out = mode == 0 && rem > 0 && rem <= kCUs / 2 && lt < 8 * kCUs && k >= 2048;
-    return Choice{cfg, even_out ? 2 : 1};
+    // FP64's large tile runs two workgroups per CU: K split in two when the
+    // last round of 2 * kCUs would be at most a quarter full (cache flushed,
+    // beta 0.5: 576 tiles 0.86-0.97x the unsplit time, 640 tiles 0.91x; 648
+    // and 672 tiles 1.04-1.07x, so they stay unsplit).
+    const int rem64 = lt % (2 * kCUs);
+    const bool even_out64 = mode == 1 && rem64 > 0 && rem64 <= kCUs / 2 && lt < 8 * kCUs && k >= 1024;
+    return Choice{cfg, (even_out || even_out64) ? 2 : 1};
   } else if (mode == 0 && lt >= kCUs / 2 && 2 * lt <= kCUs + kCUs / 4 && k >= 2048) {
     return Choice{cfg, 2};  // the large FP32 tile, K split in two, about one round
   }
