@@ -201,6 +201,21 @@ def test_fp64_vs_long_double_with_outliers(cc, dev):
         assert np.all(np.abs(got - ref) <= 4 * 300 * 2.0**-53 * absprod)
 
 
+def test_fp64_large_output_split_in_two(cc, dev):
+    """FP64 3072^3: 576 tiles of 128x128 leave the second round of 512
+    workgroups 1/8 full, so the heuristic splits K in two; the result stays
+    within 2*K*u*|A||B| (u = 2^-53) of torch's FP64 product."""
+    M = N = K = 3072
+    assert cc.heuristic(1, M, N, K) == (7, 2)
+    g = torch.Generator(device="cpu").manual_seed(21)
+    A = torch.randn(M, K, generator=g, dtype=torch.float64).to(dev)
+    B = torch.randn(K, N, generator=g, dtype=torch.float64).to(dev)
+    got = cc.gemm(A, B, mode="fp64")
+    ref = A @ B
+    bound = 2 * K * 2.0**-53 * (A.abs() @ B.abs())
+    assert torch.all((got - ref).abs() <= bound)
+
+
 @pytest.mark.parametrize("mode,dt", MODES)
 def test_alpha_beta_semantics(cc, dev, mode, dt):
     rng = np.random.default_rng(102)
