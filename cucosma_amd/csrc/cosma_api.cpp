@@ -79,12 +79,15 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    one workgroup (256x256x1048576: 256 slices of 4096, so every input byte
 //    is read from HBM once).  It beat every smaller tile from 2048^3 up, and
 //    for 4096x4096x16384 (the weight gradient of a 16384x4096x4096 layer) ran
-//    at twice the speed of 128x128.  When M*N/(M+N) >= 3072 (8192^3,
-//    16384x4096, 12288^3, ...) the 256x256 tile runs over pre-split operands
-//    (planar_cfg): the split pass costs (M+N)*K*8 bytes of HBM traffic, and
-//    the GEMM without the in-loop split is 16% faster with 8 waves, 22% with
-//    16 (8192^3: 2.35 ms in total against 2.64 for cfg 6; 4096^3 and
-//    2048x8192x2048 lose to cfg 6, which they keep).  The pre-split tile is
+//    at twice the speed of 128x128.  The 256x256 tile runs over pre-split
+//    operands (planar_cfg) when its rounds of workgroups (ceil(tiles / CUs))
+//    times 5376 reach M + N: the split pass costs (M+N)*K*8 bytes of HBM
+//    traffic, the GEMM without the in-loop split saves 16-22% of each round,
+//    and both scale with K (8192^3: 2.35 ms in total against 2.64 for cfg 6;
+//    4608^2x4096, 5120^2x4096, 4352^3 and 4096x6144x4096, with a partly
+//    filled second round, 6-8% faster in both split modes; 4096^3,
+//    2048x8192x2048 and 4096x8192x4096 tie or lose, and keep cfg 6; 5632^2x2048
+//    lost 1% and is just outside).  The pre-split tile is
 //    also taken when its 256x256 tiles number between half the CUs and all
 //    of them (one workgroup per CU, some CUs idle): there the 16-wave kernel's
 //    faster K loop outweighs the split pass (BF16x3 +2% to +17% over nine
@@ -161,7 +164,7 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
-      if (split == 1 && (lt < kCUs || (int64_t)m * n >= 3072 * ((int64_t)m + n)))
+      if (split == 1 && (lt < kCUs || (int64_t)cdiv(lt, kCUs) * 5376 >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }
