@@ -81,13 +81,14 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    for 4096x4096x16384 (the weight gradient of a 16384x4096x4096 layer) ran
 //    at twice the speed of 128x128.  The 256x256 tile runs over pre-split
 //    operands (planar_cfg) when its rounds of workgroups (ceil(tiles / CUs))
-//    times 5376 reach M + N: the split pass costs (M+N)*K*8 bytes of HBM
+//    times 5504 reach M + N: the split pass costs (M+N)*K*8 bytes of HBM
 //    traffic, the GEMM without the in-loop split saves 16-22% of each round,
 //    and both scale with K (8192^3: 2.35 ms in total against 2.64 for cfg 6;
-//    4608^2x4096, 5120^2x4096, 4352^3 and 4096x6144x4096, with a partly
-//    filled second round, 6-8% faster in both split modes; 4096^3,
-//    2048x8192x2048 and 4096x8192x4096 tie or lose, and keep cfg 6; 5632^2x2048
-//    lost 1% and is just outside).  The pre-split tile is
+//    4608^2x4096, 5120^2x4096, 4352^3, 4096x6144x4096 and 4096x12288x4096,
+//    with partly filled or few rounds, 6-8% faster in both split modes;
+//    4096^3, 2048x8192x2048 and 4096x8192x4096 tie or lose, and keep cfg 6;
+//    5632^2x2048 lost 1% and is just outside).  5504 >= 2^24 / 3072, so every
+//    output with M*N/(M+N) >= 3072 also qualifies.  The pre-split tile is
 //    also taken when its 256x256 tiles number between half the CUs and all
 //    of them (one workgroup per CU, some CUs idle): there the 16-wave kernel's
 //    faster K loop outweighs the split pass (BF16x3 +2% to +17% over nine
@@ -164,7 +165,7 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
-      if (split == 1 && (lt < kCUs || (int64_t)cdiv(lt, kCUs) * 5376 >= (int64_t)m + n))
+      if (split == 1 && (lt < kCUs || (int64_t)cdiv(lt, kCUs) * 5504 >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }
