@@ -152,7 +152,12 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   if (split > max_split) split = max_split;
   return split < 1 ? 1 : split;
 }
-constexpr Choice heuristic(int mode, int m, int n, int k) {
+// ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A
+// in their loaders; the pre-split pass does it once, so with a transposed A the
+// pre-split tile wins even at one round of workgroups (BF16x3 4096x4096x16384,
+// the weight gradient of a 16384x4096x4096 layer: 1.185 ms against 1.353 for
+// cfg 6; BF16x4 1.562 against 1.606).
+constexpr Choice heuristic(int mode, int m, int n, int k, int ta = 0) {
   int cfg = large_cfg(mode);
   const int lt = tiles_of(mode, cfg, m, n);
   if (split_mode(mode)) {
@@ -165,7 +170,7 @@ constexpr Choice heuristic(int mode, int m, int n, int k) {
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
-      if (split == 1 && (lt < kCUs || (int64_t)cdiv(lt, kCUs) * 5504 >= (int64_t)m + n))
+      if (split == 1 && (lt < kCUs || ta || (int64_t)cdiv(lt, kCUs) * 5504 >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }
@@ -387,7 +392,7 @@ int validate(const cosma_gemm_desc* d) {
 // Tile configuration, split factor and group size of a validated call with
 // K > 0: the caller's, else the tuned table's, else the heuristic's.
 void resolve_choice(const cosma_gemm_desc* d, int* cfg, int* split, int* group_m) {
-  Choice ch = heuristic(d->mode, d->m, d->n, d->k);
+  Choice ch = heuristic(d->mode, d->m, d->n, d->k, d->trans_a);
   *group_m = d->group_m;
   TuneVal tv;
   if (d->tile_cfg < 0 && d->split_k <= 0 &&
