@@ -156,7 +156,9 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
 // in their loaders; the pre-split pass does it once, so with a transposed A the
 // pre-split tile wins even at one round of workgroups (BF16x3 4096x4096x16384,
 // the weight gradient of a 16384x4096x4096 layer: 1.185 ms against 1.353 for
-// cfg 6; BF16x4 1.562 against 1.606).
+// cfg 6; BF16x4 1.562 against 1.606; 4096^2x1024 0.91x, 6144^2x1024 0.77x,
+// 2048x8192x2048 0.89x the cfg 6 time).  Not for K < 512, where the split
+// pass and its launches outweigh it (4096^2x256: 1.15x).
 constexpr Choice heuristic(int mode, int m, int n, int k, int ta = 0) {
   int cfg = large_cfg(mode);
   const int lt = tiles_of(mode, cfg, m, n);
@@ -170,7 +172,7 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta = 0) {
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
-      if (split == 1 && (lt < kCUs || ta || (int64_t)cdiv(lt, kCUs) * 5504 >= (int64_t)m + n))
+      if (split == 1 && (lt < kCUs || (ta && k >= 512) || (int64_t)cdiv(lt, kCUs) * 5504 >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }
