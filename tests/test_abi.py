@@ -72,6 +72,47 @@ def test_tile_table_and_heuristic():
     assert cucosma_amd.workspace_bytes(256, 256, 1 << 20, split_k=1) == 0
 
 
+def test_heuristic_round_rules():
+    """The split and tile rules that count rounds of workgroups (no GPU: the
+    heuristic is host code; measurements behind each rule are in the
+    library's comments)."""
+    import cucosma_amd
+
+    h = cucosma_amd.heuristic
+    # FP64 medium outputs: smallest split whose last round (two 128x64 or four
+    # 64x64 workgroups per CU) is at least three quarters full.
+    assert h(1, 1024, 2048, 2048) == (2, 2)   # 256 tiles: split 3 would leave 1/2 of a second round
+    assert h(1, 2048, 1024, 1024) == (2, 2)
+    assert h(1, 1792, 1792, 1792) == (2, 1)   # 392 tiles: 0.77 of one round
+    assert h(1, 1536, 1536, 1536) == (2, 3)
+    assert h(1, 1024, 1024, 1024) == (1, 3)
+    # FP64 large outputs: split in two when the last round of 512 is at most 1/4 full.
+    assert h(1, 3072, 3072, 3072) == (7, 2)   # 576 tiles
+    assert h(1, 4608, 2304, 2048) == (7, 1)   # 648 tiles: 136 of 512
+    assert h(1, 4096, 4096, 4096) == (7, 1)   # whole rounds
+    # BF16: pre-split tile when ceil(tiles / 256) * 5504 >= M + N.
+    for mode in (2, 3):
+        assert h(mode, 4608, 4608, 4096) == (8, 1)
+        assert h(mode, 4096, 12288, 4096) == (8, 1)
+        assert h(mode, 4096, 4096, 4096) == (6, 1)
+        assert h(mode, 5632, 5632, 2048) == (6, 1)
+        assert h(mode, 8192, 8192, 8192) == (8, 1)
+
+
+def test_transposed_a_takes_the_presplit_tile():
+    """With A stored transposed (K >= 512) the split modes take the pre-split
+    tile, so the call needs workspace for the planes: (M + N) rows of 2 * K
+    bf16.  Row-major A at one round keeps the on-the-fly tile (no workspace)."""
+    import cucosma_amd
+
+    for mode in (2, 3):
+        for ta, k, want in ((1, 16384, (4096 + 4096) * 2 * 16384 * 2), (0, 16384, 0), (1, 256, 0)):
+            d = _desc(mode=mode, m=4096, n=4096, k=k, trans_a=ta, lda=4096 if ta else k, ldb=4096, ldc=4096)
+            out = C.c_size_t()
+            assert cucosma_amd.lib.cosma_workspace_query(C.byref(d), C.byref(out)) == 0
+            assert out.value == want, (mode, ta, k, out.value)
+
+
 def _desc(**kw):
     import cucosma_amd
     from cucosma_amd._lib import GemmDesc
