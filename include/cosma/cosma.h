@@ -125,7 +125,9 @@ int cosma_workspace_set(void *ptr, size_t bytes);
  * synchronising the device (the memory cudaFree returns at the end of every
  * reference call, cuCOSMAV100.cuh:2838).  Buffers a captured graph uses are
  * kept.  *freed_bytes (may be NULL) receives the bytes freed.  Not while any
- * stream is capturing. */
+ * stream is capturing, nor while another thread may be inside a GEMM call on
+ * this device (a call that has picked its buffer but not yet launched would
+ * then use freed memory). */
 int cosma_workspace_release(size_t *freed_bytes);
 
 /* Pre-split operands (BF16X3 / BF16X4).  The split modes can split each FP32
