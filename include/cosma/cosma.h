@@ -24,6 +24,9 @@
  *     atomic variant, so results are bitwise repeatable.
  *   - Errors are returned, never abort the process.
  *   - Calls take an optional stream; NULL is the default stream.
+ *   - Threads: calls on different streams may be made from different threads
+ *     at once; calls that use one stream must not overlap on the host (a call
+ *     that grows the stream's workspace frees the previous buffer).
  *
  * Status codes: 0 success; negative = contract violation detected on the host
  * before any GPU work; positive = the hipError_t reported by the HIP runtime.
