@@ -13,7 +13,9 @@ from ._lib import (  # noqa: F401
     EPILOGUES,
     CosmaError,
     check,
+    device_topology,
     heuristic,
+    peak_tflops,
     lib,
     tile_configs,
 )
@@ -33,7 +35,7 @@ from .gemm import (  # noqa: F401
     workspace_bytes,
 )
 
-from .grouped import GroupedGemm, grouped_gemm  # noqa: F401,E402
+from .grouped import GroupedGemm, group_families, grouped_autotune, grouped_gemm  # noqa: F401,E402
 from .autograd import CosmaMatmul, matmul  # noqa: F401,E402
 
 __version__ = lib.cosma_version().decode()

@@ -87,6 +87,18 @@ class GroupProblem(C.Structure):
     ]
 
 
+class GroupTuneResult(C.Structure):
+    """Mirror of struct cosma_group_tune_result."""
+
+    _fields_ = [
+        ("bucket", C.c_int),
+        ("tile_cfg", C.c_int),
+        ("split_k", C.c_int),
+        ("status", C.c_int),
+        ("ms", C.c_double),
+    ]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -125,6 +137,8 @@ def _load():
         "cosma_num_tile_configs": (i, [i]),
         "cosma_tile_config": (i, [i, i, C.POINTER(i)]),
         "cosma_heuristic": (i, [i, i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "cosma_heuristic_topo": (i, [i, i, i, i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "cosma_device_topology": (i, [C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "cosma_error_stats_f32": (i, [i, i, vp, i64, vp, i64, f64p, vp]),
         "cosma_error_stats_f64": (i, [i, i, vp, i64, vp, i64, f64p, vp]),
         "cosma_sumsq_f32": (i, [i, i, vp, i64, f64p, vp]),
@@ -132,6 +146,12 @@ def _load():
         "cosma_convert_f32_f64": (i, [i, i, vp, i64, vp, i64, vp]),
         "cosma_grouped_plan_create": (i, [i, i, C.POINTER(GroupProblem), C.POINTER(i), i,
                                           C.POINTER(vp)]),
+        "cosma_grouped_plan_create_ex": (i, [i, i, C.POINTER(GroupProblem), C.POINTER(i), C.POINTER(i), i,
+                                             C.POINTER(vp)]),
+        "cosma_grouped_plan_choice": (i, [vp, i, C.POINTER(i), C.POINTER(i)]),
+        "cosma_group_families": (i, [i, i, C.POINTER(i)]),
+        "cosma_grouped_autotune": (i, [i, i, C.POINTER(GroupProblem), i, i, C.POINTER(i), C.POINTER(i),
+                                       C.POINTER(GroupTuneResult), i, C.POINTER(i)]),
         "cosma_grouped_plan_run": (i, [vp, vp]),
         "cosma_grouped_plan_run_graph": (i, [vp, vp]),
         "cosma_grouped_plan_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
@@ -177,7 +197,40 @@ def tile_configs(mode: int) -> list[dict]:
     return out
 
 
-def heuristic(mode: int, m: int, n: int, k: int) -> tuple[int, int]:
+def heuristic(mode: int, m: int, n: int, k: int, *, cus: int | None = None,
+              trans_a: bool = False) -> tuple[int, int]:
+    """(tile_cfg, split_k) the library picks for a shape: on the current
+    device, or on a device of `cus` compute units."""
     cfg, split = C.c_int(), C.c_int()
-    check(lib.cosma_heuristic(mode, m, n, k, C.byref(cfg), C.byref(split)), "cosma_heuristic")
+    if cus is None and not trans_a:
+        check(lib.cosma_heuristic(mode, m, n, k, C.byref(cfg), C.byref(split)), "cosma_heuristic")
+    else:
+        if cus is None:
+            cus = device_topology()["cus"]
+        check(lib.cosma_heuristic_topo(mode, m, n, k, int(trans_a), int(cus), C.byref(cfg), C.byref(split)),
+              "cosma_heuristic_topo")
     return cfg.value, split.value
+
+
+def device_topology() -> dict:
+    """Compute units, XCDs and peak engine clock (kHz) of the current device;
+    queried=False means no device answered and the nominal MI355X figures
+    stand in."""
+    cus, xcds, clk = C.c_int(), C.c_int(), C.c_int()
+    st = lib.cosma_device_topology(C.byref(cus), C.byref(xcds), C.byref(clk))
+    return dict(cus=cus.value, xcds=xcds.value, clock_khz=clk.value, queried=st == 0)
+
+
+# Dense MFMA flops per clock per CU on gfx950 (FP32 v_mfma_f32_16x16x4_f32,
+# FP64 v_mfma_f64_16x16x4_f64, BF16 v_mfma_f32_16x16x32_bf16).
+MFMA_FLOPS_PER_CLK_CU = {"fp32": 256, "fp64": 128, "bf16": 4096}
+
+
+def peak_tflops(kind: str, topo: dict | None = None) -> float:
+    """Dense MFMA peak of `kind` (fp32, fp64, bf16, bf16x3 = bf16/3 useful,
+    bf16x4 = bf16/4) from the device's CU count and peak clock: the published
+    MI355X figures (157.3 / 78.6 / 2516.6 TFLOP/s) for 256 CUs at 2.4 GHz."""
+    topo = topo or device_topology()
+    div = {"bf16x3": 3, "bf16x4": 4}.get(kind, 1)
+    base = MFMA_FLOPS_PER_CLK_CU["bf16" if kind.startswith("bf16") else kind]
+    return base * topo["cus"] * topo["clock_khz"] * 1e3 / div / 1e12

@@ -6,13 +6,18 @@ reference's own thesis reports that volume is the wrong metric and that
 occupancy and L2 behaviour decide (thesis p41-42, p44), so this tuner
 measures instead:
 
-* ``tune()`` runs the C++ tuner (``cosma_autotune``): every built tile
-  configuration x split-K factor, then XCD group sizes for the winner, timed
-  with HIP events; each candidate carries its occupancy (workgroups per CU,
-  waves per SIMD) and MFMA utilisation (achieved / dense MFMA peak).
-* ``counters()`` re-runs chosen candidates under ``rocprofv3 --pmc`` and adds
-  L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)) and HBM bytes (FETCH_SIZE,
-  WRITE_SIZE) to the record.
+* ``tune()`` runs the C++ tuner (``cosma_autotune_ex``): every built tile
+  configuration x split-K factor, then XCD group sizes for the winner, each
+  timed call after a 512 MiB cache flush (the benchmark protocol) with HIP
+  events; each candidate carries its occupancy (workgroups per CU, waves per
+  SIMD) and achieved fraction of the mode's dense MFMA peak.
+* ``counters()`` re-runs a candidate under ``rocprofv3 --pmc`` and measures
+  its MFMA busy share (SQ_VALU_MFMA_BUSY_CYCLES over busy SIMD cycles), L2
+  hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)) and HBM bytes (FETCH_SIZE,
+  WRITE_SIZE); ``tune(with_counters=True)`` attaches them to the best
+  candidate's record.
+* ``cucosma_amd.grouped_autotune`` (``cosma_grouped_autotune``) picks tile
+  configuration and split-K per size bucket of a grouped batch.
 * ``TuneDB`` persists results as JSON, installs them into the library
   (``cosma_tune_set``) so every C-ABI call of that shape uses them, and
   imports / exports the reference's 22-column schedule CSV
@@ -48,7 +53,7 @@ class TuneResult(C.Structure):
 
     _fields_ = [
         ("tile_cfg", C.c_int), ("split_k", C.c_int), ("group_m", C.c_int), ("status", C.c_int),
-        ("ms", C.c_double), ("tflops", C.c_double), ("mfma_util", C.c_double),
+        ("ms", C.c_double), ("tflops", C.c_double), ("achieved_frac", C.c_double),
         ("blocks", C.c_int), ("occupancy", C.c_int), ("waves_per_simd", C.c_int),
         ("workspace_bytes", C.c_size_t),
     ]
@@ -57,6 +62,9 @@ class TuneResult(C.Structure):
 lib.cosma_autotune.restype = C.c_int
 lib.cosma_autotune.argtypes = [C.POINTER(GemmDesc), C.c_int, C.c_int, C.POINTER(TuneResult),
                                C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+lib.cosma_autotune_ex.restype = C.c_int
+lib.cosma_autotune_ex.argtypes = [C.POINTER(GemmDesc), C.c_int, C.c_int, C.c_size_t, C.POINTER(TuneResult),
+                                  C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
 lib.cosma_tune_set.restype = C.c_int
 lib.cosma_tune_set.argtypes = [C.c_int] * 9
 lib.cosma_tune_lookup.restype = C.c_int
@@ -71,8 +79,12 @@ def _key(mode: str, m: int, n: int, k: int, ta: bool, tb: bool) -> str:
 
 def tune(mode: str, m: int, n: int, k: int, *, trans_a: bool = False, trans_b: bool = False,
          warmup: int = 2, reps: int = 5, alpha: float = 1.0, beta: float = 0.0,
-         tile_cfg: int = -1, split_k: int = 0, group_m: int = 0) -> dict:
-    """Time every candidate for one shape on random operands; return the record."""
+         tile_cfg: int = -1, split_k: int = 0, group_m: int = 0, flush: bool = True,
+         with_counters: bool = False) -> dict:
+    """Time every candidate for one shape on random operands (caches flushed
+    before each timed call unless flush is False); return the record, the
+    fastest first.  with_counters adds the best candidate's measured MFMA busy
+    share, L2 hit rate and HBM bytes (counters(), rocprofv3)."""
     import torch
 
     from .gemm import _ld, _stream_ptr
@@ -95,8 +107,8 @@ def tune(mode: str, m: int, n: int, k: int, *, trans_a: bool = False, trans_b: b
     cap = 512
     res = (TuneResult * cap)()
     nres, best = C.c_int(), C.c_int()
-    check(lib.cosma_autotune(C.byref(d), warmup, reps, res, cap, C.byref(nres), C.byref(best)),
-          "cosma_autotune")
+    check(lib.cosma_autotune_ex(C.byref(d), warmup, reps, (512 << 20) if flush else 0, res, cap,
+                                C.byref(nres), C.byref(best)), "cosma_autotune_ex")
     tiles = tile_configs(MODES[mode])
     cands = []
     for i in range(nres.value):
@@ -104,18 +116,41 @@ def tune(mode: str, m: int, n: int, k: int, *, trans_a: bool = False, trans_b: b
         t = tiles[r.tile_cfg]
         cands.append(dict(tile_cfg=r.tile_cfg, tile=f"{t['BM']}x{t['BN']}x{t['BK']}/{t['WM']}x{t['WN']}",
                           split_k=r.split_k, group_m=r.group_m, status=r.status,
-                          ms=r.ms, tflops=r.tflops, mfma_util=r.mfma_util, blocks=r.blocks,
+                          ms=r.ms, tflops=r.tflops, achieved_frac=r.achieved_frac, blocks=r.blocks,
                           occupancy=r.occupancy, waves_per_simd=r.waves_per_simd,
                           workspace_bytes=int(r.workspace_bytes)))
-    rec = dict(mode=mode, m=m, n=n, k=k, trans_a=bool(trans_a), trans_b=bool(trans_b),
+    rec = dict(mode=mode, m=m, n=n, k=k, trans_a=bool(trans_a), trans_b=bool(trans_b), flushed=bool(flush),
                candidates=cands, best=cands[best.value] if best.value >= 0 else None)
+    if with_counters and rec["best"]:
+        b = rec["best"]
+        cnt = counters(mode, m, n, k, b["tile_cfg"], b["split_k"], b["group_m"], trans_a, trans_b)
+        b["counters"] = cnt
+        b["mfma_busy"] = cnt.get("mfma_busy")
+        b["l2_hit"] = cnt.get("l2_hit")
     return rec
 
 
+# Counter passes of counters(): each within one pass's hardware limits (TCC:
+# 4 counters, FETCH_SIZE takes 3 and WRITE_SIZE 2; SQ 8; GRBM 2).
+COUNTER_PASSES = (["TCC_HIT_sum", "TCC_MISS_sum"], ["FETCH_SIZE"], ["WRITE_SIZE"],
+                  ["SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE"])
+
+
 def counters(mode: str, m: int, n: int, k: int, tile_cfg: int, split_k: int, group_m: int = 8,
-             trans_a: bool = False, trans_b: bool = False, reps: int = 2) -> dict:
-    """L2 hit rate and HBM bytes of one candidate, measured by rocprofv3 in a
-    child process (one pass per counter group; needs rocprofv3 and a GPU)."""
+             trans_a: bool = False, trans_b: bool = False, reps: int = 2, timeout: float = 120.0) -> dict:
+    """Hardware counters of one candidate's main kernel (gemm_kernel or
+    planar_kernel), measured by rocprofv3 in a child process, one pass per
+    counter group, averaged over the candidate's launches:
+
+    * l2_hit: TCC_HIT / (TCC_HIT + TCC_MISS);
+    * fetch_kib / write_kib: FETCH_SIZE / WRITE_SIZE per launch (KiB);
+    * mfma_busy: SQ_VALU_MFMA_BUSY_CYCLES / (cycles the GPU was busy x
+      SIMDs), with GRBM_GUI_ACTIVE summed over the XCDs (per-XCD cycles =
+      GRBM_GUI_ACTIVE / XCDs) and 4 SIMDs per CU -- the share of SIMD cycles
+      the matrix cores were busy, whatever the clock.
+    Needs rocprofv3 and a GPU."""
+    from ._lib import device_topology
+
     cmd_run = [sys.executable, "-m", "cucosma_amd.autotune", "--run-one", "--mode", mode,
                "--shape", str(m), str(n), str(k), "--tile-cfg", str(tile_cfg),
                "--split-k", str(split_k), "--group-m", str(group_m), "--reps", str(reps)]
@@ -123,27 +158,40 @@ def counters(mode: str, m: int, n: int, k: int, tile_cfg: int, split_k: int, gro
         cmd_run.append("--trans-a")
     if trans_b:
         cmd_run.append("--trans-b")
-    out = {}
-    passes = (["TCC_HIT_sum", "TCC_MISS_sum"], ["FETCH_SIZE"], ["WRITE_SIZE"])
+    topo = device_topology()
+    raw: dict[str, list[float]] = {}
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
-    for pmc in passes:
+    cwd = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for pmc in COUNTER_PASSES:
         with tempfile.TemporaryDirectory() as td:
             cmd = ["rocprofv3", "--pmc", *pmc, "--kernel-trace", "--output-format", "csv", "-d", td,
                    "-o", "run", "--", *cmd_run]
-            subprocess.run(cmd, check=True, capture_output=True, timeout=120, env=env)
-            vals = {}
+            subprocess.run(cmd, check=True, capture_output=True, timeout=timeout, env=env, cwd=cwd)
+            per: dict[tuple, dict] = {}
             for root, _, files in os.walk(td):
                 for f in files:
-                    if f.endswith("counter_collection.csv"):
-                        for row in csv.DictReader(open(os.path.join(root, f))):
-                            if "gemm_kernel" in row["Kernel_Name"]:
-                                vals.setdefault(row["Counter_Name"], []).append(
-                                    float(row["Counter_Value"]))
-            for name, v in vals.items():
-                out[name] = sum(v) / len(v)
+                    if not f.endswith("counter_collection.csv"):
+                        continue
+                    for row in csv.DictReader(open(os.path.join(root, f))):
+                        name = row["Kernel_Name"]
+                        if "gemm_kernel" not in name and "planar_kernel" not in name:
+                            continue
+                        key = (row.get("Dispatch_Id", ""), name)
+                        per.setdefault(key, {})[row["Counter_Name"]] = float(row["Counter_Value"])
+            for vals in per.values():
+                for c, v in vals.items():
+                    raw.setdefault(c, []).append(v)
+                if "SQ_VALU_MFMA_BUSY_CYCLES" in vals and vals.get("GRBM_GUI_ACTIVE", 0) > 0:
+                    simd_cycles = vals["GRBM_GUI_ACTIVE"] / max(topo["xcds"], 1) * topo["cus"] * 4
+                    raw.setdefault("mfma_busy", []).append(vals["SQ_VALU_MFMA_BUSY_CYCLES"] / simd_cycles)
+    out = {c: sum(v) / len(v) for c, v in raw.items() if v}
     if "TCC_HIT_sum" in out and "TCC_MISS_sum" in out:
         tot = out["TCC_HIT_sum"] + out["TCC_MISS_sum"]
-        out["l2_hit_rate"] = out["TCC_HIT_sum"] / tot if tot else None
+        out["l2_hit"] = out["TCC_HIT_sum"] / tot if tot else None
+    if "FETCH_SIZE" in out:
+        out["fetch_kib"] = out["FETCH_SIZE"]
+    if "WRITE_SIZE" in out:
+        out["write_kib"] = out["WRITE_SIZE"]
     return out
 
 
@@ -275,6 +323,8 @@ def main(argv=None) -> int:
             b = rec["best"]
             b["counters"] = counters(args.mode, m, n, k, b["tile_cfg"], b["split_k"], b["group_m"],
                                      args.trans_a, args.trans_b)
+            b["mfma_busy"] = b["counters"].get("mfma_busy")
+            b["l2_hit"] = b["counters"].get("l2_hit")
         db.add(rec)
         b = rec["best"]
         print(json.dumps(dict(mode=args.mode, m=m, n=n, k=k, best=b,

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -34,7 +35,13 @@ namespace {
 using cosma::Problem;
 
 constexpr int kMaxSplit = 4096;
-constexpr int kCUs = 256;  // MI355X compute units
+constexpr int kMaxGroupM = 65535;  // tile rows per grouped-order group (map_block works in int)
+// Topology the fixed-shape entry points are specialised for at compile time:
+// a whole MI355X (256 CUs in 8 XCDs).  Every other choice uses the topology
+// the current device reports (device_topo()); a fixed-shape call on a device
+// with another CU count recomputes its choice for that device.
+constexpr int kNominalCUs = 256;
+constexpr int kNominalXcds = 8;
 
 // Tile extents of the built configurations, mirrored as constants so the
 // fixed-shape entry points can pick them at compile time.  Must match the
@@ -43,23 +50,23 @@ constexpr int kCUs = 256;  // MI355X compute units
 struct TileDims {
   int bm, bn, bk;
 };
-constexpr int kMaxCfg = 9;
+constexpr int kMaxCfg = 12;
 constexpr TileDims kDims[4][kMaxCfg] = {
-    // FP32
+    // FP32 (cfgs 8, 9: thin and small variants of the 16-wave tile for grouped launches)
     {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}, {256, 128, 32},
-     {256, 128, 32}, {256, 128, 32}},
-    // FP64
+     {256, 128, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32}},
+    // FP64 (cfgs 8-10: variants of the 128x128 ping-pong tile)
     {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 16}, {64, 64, 32}, {128, 128, 32},
-     {256, 128, 16}, {128, 128, 16}},
-    // BF16X3 (cfg 7: pre-split operands)
+     {256, 128, 16}, {128, 128, 16}, {128, 64, 16}, {64, 128, 16}, {64, 64, 16}},
+    // BF16X3 (cfg 7, 8: pre-split operands; 9-11: variants of the 256x256 ping-pong tile)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}},
+     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32}},
     // BF16X4 (same tiles as BF16X3)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}},
+     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {8, 8, 9, 9};
+constexpr int kNumCfg[kNumModes] = {10, 11, 12, 12};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -159,71 +166,73 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
 // cfg 6; BF16x4 1.562 against 1.606; 4096^2x1024 0.91x, 6144^2x1024 0.77x,
 // 2048x8192x2048 0.89x the cfg 6 time).  Not for K < 512, where the split
 // pass and its launches outweigh it (4096^2x256: 1.15x).
-constexpr Choice heuristic(int mode, int m, int n, int k, int ta = 0) {
+constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
+  // An empty output launches nothing; a non-positive CU count is not a device.
+  if (m <= 0 || n <= 0 || cus <= 0) return Choice{medium_cfg(mode), 1};
   int cfg = large_cfg(mode);
   const int lt = tiles_of(mode, cfg, m, n);
   if (split_mode(mode)) {
     const int64_t covered = (int64_t)lt * kDims[mode][cfg].bm * kDims[mode][cfg].bn;
-    const int split = lt >= kCUs / 2 ? 1 : clamp_split(kCUs / lt, k, 512, kCUs);
+    const int split = lt >= cus / 2 ? 1 : clamp_split(cus / lt, k, 512, cus);
     // K is split over the large tile while that gives half the CUs a
     // workgroup, unless the slices end up at most 512 deep and the CUs are
     // still not all busy (then 128x128 tiles below do better).
-    const bool thin = split > 1 && (int64_t)lt * split < kCUs && k / split <= 512;
-    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= kCUs / 2 && !thin) {
+    const bool thin = split > 1 && (int64_t)lt * split < cus && k / split <= 512;
+    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= cus / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
-      if (split == 1 && (lt < kCUs || (ta && k >= 512) || (int64_t)cdiv(lt, kCUs) * 5504 >= (int64_t)m + n))
+      if (split == 1 && (lt < cus || (ta && k >= 512) || (int64_t)cdiv(lt, cus) * 5504 >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }
-  } else if (lt >= 2 * kCUs) {
-    const int rem = lt % kCUs;
-    const bool even_out = mode == 0 && rem > 0 && rem <= kCUs / 2 && lt < 8 * kCUs && k >= 2048;
+  } else if (lt >= 2 * cus) {
+    const int rem = lt % cus;
+    const bool even_out = mode == 0 && rem > 0 && rem <= cus / 2 && lt < 8 * cus && k >= 2048;
     // FP64's large tile runs two workgroups per CU: K split in two when the
-    // last round of 2 * kCUs would be at most a quarter full (cache flushed,
+    // last round of 2 * cus would be at most a quarter full (cache flushed,
     // beta 0.5: 576 tiles 0.86-0.97x the unsplit time, 640 tiles 0.91x; 648
     // and 672 tiles 1.04-1.07x, so they stay unsplit).
-    const int rem64 = lt % (2 * kCUs);
-    const bool even_out64 = mode == 1 && rem64 > 0 && rem64 <= kCUs / 2 && lt < 8 * kCUs && k >= 1024;
+    const int rem64 = lt % (2 * cus);
+    const bool even_out64 = mode == 1 && rem64 > 0 && rem64 <= cus / 2 && lt < 8 * cus && k >= 1024;
     return Choice{cfg, (even_out || even_out64) ? 2 : 1};
-  } else if (mode == 0 && lt >= kCUs / 2 && 2 * lt <= kCUs + kCUs / 4 && k >= 2048) {
+  } else if (mode == 0 && lt >= cus / 2 && 2 * lt <= cus + cus / 4 && k >= 2048) {
     return Choice{cfg, 2};  // the large FP32 tile, K split in two, about one round
   }
   const int dc = deep_cfg(mode, m, n);
   const int dtiles = tiles_of(mode, dc, m, n);
-  if (k >= 16384 && dtiles < kCUs) {
+  if (k >= 16384 && dtiles < cus) {
     const int per_cu = (mode == 1 || dc == 2) ? 2 : 1;
-    return Choice{dc, clamp_split(cdiv(per_cu * kCUs, dtiles), k, 512, kCUs)};
+    return Choice{dc, clamp_split(cdiv(per_cu * cus, dtiles), k, 512, cus)};
   }
-  if (mode == 1 && tiles_of(mode, 3, m, n) >= kCUs) {
+  if (mode == 1 && tiles_of(mode, 3, m, n) >= cus) {
     // FP64 128x128 (copy after the MFMAs); K split in two unless the tiles
     // make whole rounds of CUs and K is short.
     const int t3 = tiles_of(mode, 3, m, n);
-    return Choice{3, (t3 % kCUs == 0 && k < 4096) ? 1 : 2};
+    return Choice{3, (t3 % cus == 0 && k < 4096) ? 1 : 2};
   }
   cfg = medium_cfg(mode);
-  if (tiles_of(mode, cfg, m, n) < kCUs) {
+  if (tiles_of(mode, cfg, m, n) < cus) {
     // BF16: 128x128 with K split (a power of two) toward two workgroups per
     // CU when that fills every CU with slices at least 512 deep; otherwise
     // 64x64.
     const int t0 = tiles_of(mode, cfg, m, n);
     int want = 1;
-    while (want * 2 * t0 <= 2 * kCUs) want *= 2;
+    while (want * 2 * t0 <= 2 * cus) want *= 2;
     const int s0 = clamp_split(want, k, 512);
-    if (split_mode(mode) && (int64_t)t0 * s0 >= kCUs) return Choice{cfg, s0};
+    if (split_mode(mode) && (int64_t)t0 * s0 >= cus) return Choice{cfg, s0};
     cfg = 1;  // 64x64
   }
   const int tiles = tiles_of(mode, cfg, m, n);
   int split = 1;
-  if (tiles < 2 * kCUs && k >= 1024) {
-    split = clamp_split(cdiv(3 * kCUs, tiles), k, 256);
+  if (tiles < 2 * cus && k >= 1024) {
+    split = clamp_split(cdiv(3 * cus, tiles), k, 256);
     if (mode == 1) {
       // FP64 fits two 128x64 or four 64x64 workgroups per CU, not three: take
       // the smallest split whose last round of workgroups is at least three
       // quarters full (cache flushed, beta 0.5: 1024x2048x2048 185 us at
       // split 2 against 239 at 3, 2048x1024x1024 108 against 139, 1792^3 287
       // at split 1 against 307 at 2; the other shapes tried keep their split).
-      const int64_t slots = (int64_t)(cfg == 1 ? 4 : 2) * kCUs;
+      const int64_t slots = (int64_t)(cfg == 1 ? 4 : 2) * cus;
       const int smax = clamp_split(64, k, 256);
       for (int s = 1; s <= smax; ++s) {
         const int64_t w = (int64_t)tiles * s;
@@ -235,8 +244,8 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta = 0) {
       }
     }
   } else if (mode == 0 && k >= 2048) {
-    const int rem = tiles % kCUs;  // FP32: even out a half-empty last round
-    if (rem > 0 && rem <= kCUs / 2 && tiles < 8 * kCUs) split = 2;
+    const int rem = tiles % cus;  // FP32: even out a half-empty last round
+    if (rem > 0 && rem <= cus / 2 && tiles < 8 * cus) split = 2;
   }
   return Choice{cfg, split};
 }
@@ -292,6 +301,33 @@ DevState* dev_state(int* status) {
   return &g_dev[dev];
 }
 
+// Compute units, XCDs and peak engine clock of a device, queried once per
+// device.  Without a usable device (no GPU, or the queries fail) the nominal
+// MI355X topology stands in, so the host-side heuristic stays defined.
+struct Topo {
+  int cus, xcds, clock_khz;
+  bool queried;  // false: the nominal stand-in
+};
+std::mutex g_topo_mu;
+std::vector<Topo> g_topo;
+
+Topo device_topo() {
+  const Topo nominal{kNominalCUs, kNominalXcds, 2400000, false};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nominal;
+  std::lock_guard<std::mutex> lk(g_topo_mu);
+  if ((int)g_topo.size() <= dev) g_topo.resize(dev + 1, Topo{0, 0, 0, false});
+  Topo& t = g_topo[dev];
+  if (t.cus > 0) return t;
+  int cus = 0, xcds = 0, clk = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return nominal;
+  if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || xcds <= 0) xcds = 1;
+  if (hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev) != hipSuccess || clk <= 0) clk = 0;
+  t = Topo{cus, xcds, clk, true};
+  return t;
+}
+
 bool capturing(hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
@@ -343,6 +379,34 @@ int resolve_workspace(const cosma_gemm_desc* d, size_t need, hipStream_t s, void
   return 0;
 }
 
+// Cache-flush buffer shared by the benchmark protocol and the autotuner:
+// allocated on first use and grown on demand (never while s is capturing).
+constexpr size_t kDefaultFlushBytes = (size_t)512 << 20;  // L2 (8 x 4 MB) and the 256 MB Infinity Cache
+int flush_buffer(size_t bytes, hipStream_t s, void** out) {
+  static std::mutex fmu;
+  static void* fbuf = nullptr;
+  static size_t fbytes = 0;
+  std::lock_guard<std::mutex> lk(fmu);
+  if (fbytes < bytes) {
+    if (capturing(s)) return COSMA_STATUS_WORKSPACE;
+    if (fbuf) {
+      hipError_t e = hipDeviceSynchronize();  // earlier flushes may still use it
+      if (e != hipSuccess) return (int)e;
+      (void)hipFree(fbuf);
+    }
+    fbuf = nullptr;
+    fbytes = 0;
+    hipError_t e = hipMalloc(&fbuf, bytes);
+    if (e != hipSuccess) {
+      fbuf = nullptr;
+      return (int)e;
+    }
+    fbytes = bytes;
+  }
+  *out = fbuf;
+  return 0;
+}
+
 // Tuned choices installed by cosma_tune_set (or the Python autotuner's DB).
 using TuneKey = std::tuple<int, int, int, int, int, int>;
 struct TuneVal {
@@ -366,7 +430,8 @@ int validate(const cosma_gemm_desc* d) {
   if (d->m < 0 || d->n < 0 || d->k < 0) return COSMA_STATUS_INVALID_VALUE;
   if ((d->trans_a != 0 && d->trans_a != 1) || (d->trans_b != 0 && d->trans_b != 1))
     return COSMA_STATUS_INVALID_VALUE;
-  if (d->split_k > kMaxSplit) return COSMA_STATUS_INVALID_VALUE;
+  if (d->split_k > kMaxSplit || d->group_m > kMaxGroupM) return COSMA_STATUS_INVALID_VALUE;
+  if ((uintptr_t)d->workspace % 16) return COSMA_STATUS_MISALIGNED;
   if (d->tile_cfg >= kNumCfg[d->mode]) return COSMA_STATUS_INVALID_VALUE;
   if (d->epilogue < COSMA_EPILOGUE_NONE || d->epilogue > COSMA_EPILOGUE_SIGMOID)
     return COSMA_STATUS_INVALID_VALUE;
@@ -394,7 +459,7 @@ int validate(const cosma_gemm_desc* d) {
 // Tile configuration, split factor and group size of a validated call with
 // K > 0: the caller's, else the tuned table's, else the heuristic's.
 void resolve_choice(const cosma_gemm_desc* d, int* cfg, int* split, int* group_m) {
-  Choice ch = heuristic(d->mode, d->m, d->n, d->k, d->trans_a);
+  Choice ch = heuristic(d->mode, d->m, d->n, d->k, d->trans_a, device_topo().cus);
   *group_m = d->group_m;
   TuneVal tv;
   if (d->tile_cfg < 0 && d->split_k <= 0 &&
@@ -444,6 +509,7 @@ int run(const cosma_gemm_desc* d) {
   p.cfg = cfg;
   p.split = split;
   p.group_m = group_m;
+  p.xcds = device_topo().xcds;
   p.act = d->epilogue;
   p.ws = nullptr;
   p.planes = nullptr;
@@ -459,14 +525,7 @@ int run(const cosma_gemm_desc* d) {
     if (split > 1) p.ws = ws;
     if (is_planar(d->mode, cfg)) p.planes = static_cast<char*>(ws) + planes_off;
   }
-  hipError_t e;
-  switch (d->mode) {
-    case COSMA_MODE_FP32: e = cosma::run_gemm_f32(p); break;
-    case COSMA_MODE_FP64: e = cosma::run_gemm_f64(p); break;
-    case COSMA_MODE_BF16X3: e = cosma::run_gemm_bf16x3(p); break;
-    default: e = cosma::run_gemm_bf16x4(p); break;
-  }
-  return (int)e;
+  return (int)cosma::run_gemm(p);
 }
 
 cosma_gemm_desc make_desc(int mode, int m, int n, int k, double alpha, const void* A, int lda,
@@ -495,9 +554,11 @@ template <int MODE, int M_, int N_, int K_>
 int fixed_gemm(double alpha, const void* A, int lda, const void* B, int ldb, double beta, void* C,
                int ldc) {
   static_assert(M_ > 0 && N_ > 0 && K_ > 0, "fixed shapes are positive");
-  constexpr Choice ch = heuristic(MODE, M_, N_, K_);
-  static_assert(ch.cfg >= 0 && ch.split >= 1, "heuristic");
+  constexpr Choice nominal = heuristic(MODE, M_, N_, K_, 0, kNominalCUs);
+  static_assert(nominal.cfg >= 0 && nominal.split >= 1, "heuristic");
   cosma_gemm_desc d = make_desc(MODE, M_, N_, K_, alpha, A, lda, B, ldb, beta, C, ldc, nullptr);
+  const int cus = device_topo().cus;
+  const Choice ch = cus == kNominalCUs ? nominal : heuristic(MODE, M_, N_, K_, 0, cus);
   d.tile_cfg = ch.cfg;
   d.split_k = ch.split;
   return run(&d);
@@ -530,18 +591,54 @@ int finish_stats(hipError_t e, const double* dev_out, double* out5, hipStream_t 
 // Per-tile cost beyond its K-steps in the grouped schedule (prologue load
 // and C store), in K-steps.
 constexpr int kTileOverheadSteps = 2;
+// Shortest K slice (in K chunks) the default grouped split produces.
+constexpr int kMinSliceSteps = 8;
 
-// Default tile of a grouped plan: one configuration for the whole batch, so
-// the plan is a single persistent launch balanced by one schedule (separate
-// buckets run as concurrent launches that compete for CUs without
-// coordination).  The large tile when outputs with a side of at least 256
-// hold half the batch's flops, else 128x128 when outputs with a side of at
-// least 128 do, else 64x64.  On 512 problems with sides log-uniform in
-// [32, 2048] the one large-tile launch ran at 76 / 44 / 118 / 110 TFLOP/s
-// (FP32 / FP64 / BF16x3 / BF16x4) against 59 / 22 / 94 / 88 for per-size
-// buckets of 64x64, 128x128 and large tiles, and within 2% of the best of
-// every combination of those tiles over three size classes.
-int group_default_cfg(int mode, int count, const cosma_group_problem* problems) {
+// Family of a tile configuration and its variant index within it.
+bool family_of(int mode, int cfg, int* fam, int* variant) {
+  int nf = 0;
+  const cosma::GroupFamily* f = cosma::group_families(mode, &nf);
+  for (int i = 0; i < nf; ++i)
+    for (int v = 0; v < f[i].ncfg; ++v)
+      if (f[i].cfgs[v] == cfg) {
+        *fam = i;
+        *variant = v;
+        return true;
+      }
+  return false;
+}
+
+// Padded output area of (m, n) on configuration cfg's tiles.
+int64_t padded_area(int mode, int cfg, int m, int n) {
+  const TileDims td = kDims[mode][cfg];
+  return (int64_t)cdiv(m, td.bm) * td.bm * (int64_t)cdiv(n, td.bn) * td.bn;
+}
+
+// Relative cost per padded MFMA flop of each variant of a mode's first tile
+// family, measured on one MI355X with the 512-problem batch (sides
+// log-uniform in [32, 2048]) run entirely on each variant: padded flops /
+// time, the first variant's rate over the variant's.  BF16x3's 8-wave thin
+// tiles read more LDS and global bytes per MFMA than the 256x256 tile's
+// 128x64 waves (padded rate 300 against 217-221 TFLOP/s, 128x128 162).
+constexpr double kVariantWeight[kNumModes][4] = {
+    {1.00, 1.12, 1.08, 1.00},  // FP32: 256x128, 128x256, 128x128 (16 waves)
+    {1.00, 1.13, 1.16, 1.27},  // FP64: 128x128, 128x64, 64x128, 64x64
+    {1.00, 1.38, 1.35, 1.85},  // BF16X3: 256x256, 256x128, 128x256, 128x128
+    {1.00, 1.13, 1.17, 1.59},  // BF16X4: the same tiles
+};
+
+// Default tile configuration of every problem of a grouped plan.
+//  - Batches whose flops sit mostly in outputs with both sides below 128
+//    (outputs with a side of at least 128 hold less than half the flops) run
+//    on 64x64 tiles, and batches whose flops sit mostly below 256 on
+//    128x128 tiles, each as one launch;
+//  - otherwise every problem runs in the mode's first tile family (the large
+//    tile and its thin and small variants: BF16 256x256 / 256x128 / 128x256 /
+//    128x128, FP32 256x128 / 128x256 / 128x128, FP64 128x128 / 128x64 /
+//    64x128 / 64x64), each problem on the variant with the least padded
+//    output area times the variant's measured cost per padded flop
+//    (kVariantWeight).  The autotuner refines this per size bucket.
+void group_default_cfgs(int mode, int count, const cosma_group_problem* problems, std::vector<int>& cfg) {
   double all = 0.0, from256 = 0.0, from128 = 0.0;
   for (int i = 0; i < count; ++i) {
     const cosma_group_problem& q = problems[i];
@@ -551,10 +648,63 @@ int group_default_cfg(int mode, int count, const cosma_group_problem* problems) 
     if (side >= 256) from256 += f;
     if (side >= 128) from128 += f;
   }
-  // (BF16x4: the copy-between-MFMAs version of the 256x256 tile, 5% above
-  // the ping-pong one in one launch over the 512 problems.)
-  if (2.0 * from256 >= all) return mode == 3 ? 3 : large_cfg(mode);
-  return 2.0 * from128 >= all ? 0 : 1;
+  cfg.assign(count, 0);
+  if (2.0 * from128 < all) {
+    std::fill(cfg.begin(), cfg.end(), 1);
+    return;
+  }
+  if (2.0 * from256 < all) {
+    std::fill(cfg.begin(), cfg.end(), 0);
+    return;
+  }
+  int nf = 0;
+  const cosma::GroupFamily& fam = cosma::group_families(mode, &nf)[0];
+  for (int i = 0; i < count; ++i) {
+    const cosma_group_problem& q = problems[i];
+    double best = 0.0;
+    for (int v = 0; v < fam.ncfg; ++v) {
+      const int c = fam.cfgs[v];
+      const double cost = (double)padded_area(mode, c, q.m, q.n) * kVariantWeight[mode][v];
+      if (v == 0 || cost < best) {
+        best = cost;
+        cfg[i] = c;
+      }
+    }
+  }
+}
+
+// Default split-K factor of every problem: a problem is split only when one
+// of its tiles would be a long pole -- its cost (K chunks plus the per-tile
+// overhead, scaled by tile area) more than twice the average load a
+// persistent workgroup gets -- into slices of about that average load, at
+// least kMinSliceSteps chunks deep.  On 512 problems with sides in
+// [32, 2048] no problem qualifies; four 256x256x65536 problems split 64 ways
+// each (256 slices for 256 CUs).
+void group_default_splits(int mode, const std::vector<const cosma_group_problem*>& probs,
+                          const std::vector<int>& cfg, int slots, std::vector<int>& split) {
+  const size_t n = probs.size();
+  split.assign(n, 1);
+  std::vector<double> cost(n, 0.0);
+  double total = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const cosma_group_problem& q = *probs[i];
+    if (q.k <= 0 || q.alpha == 0.0) continue;
+    const TileDims td = kDims[mode][cfg[i]];
+    cost[i] = (double)(cdiv(q.k, td.bk) + kTileOverheadSteps) * td.bm * td.bn;
+    total += cost[i] * cdiv(q.m, td.bm) * (double)cdiv(q.n, td.bn);
+  }
+  if (slots < 1) slots = 1;
+  for (size_t i = 0; i < n; ++i) {
+    const cosma_group_problem& q = *probs[i];
+    if (cost[i] <= 0.0) continue;
+    const TileDims td = kDims[mode][cfg[i]];
+    const double target = std::max(total / slots, (double)kMinSliceSteps * td.bm * td.bn);
+    if (cost[i] <= 2.0 * target) continue;
+    const int nk = cdiv(q.k, td.bk);
+    int s = (int)std::ceil(cost[i] / target);
+    s = std::min(s, std::max(1, nk / kMinSliceSteps));
+    split[i] = std::max(1, std::min(s, kMaxSplit));
+  }
 }
 
 }  // namespace
@@ -562,14 +712,19 @@ int group_default_cfg(int mode, int count, const cosma_group_problem* problems) 
 struct cosma_grouped_plan_s {
   int mode;
   struct Bucket {
-    int cfg;
-    int count;
-    int tiles;
-    int blocks;
-    void* table;        // device GroupEntry array, then the schedule
-    const int* sched;   // blocks + 1 offsets, then (problem, tile) pairs (inside table's allocation)
+    int fam;         // tile family (index into the mode's families)
+    int count;       // problems
+    int tiles;       // tiles over all K slices
+    int blocks;      // persistent workgroups
+    int nred;        // split members
+    int red_blocks;  // blocks of their reduction
+    void* mem;       // device: entry table | schedule | reduction table | partials
+    const void* table;
+    const int* sched;
+    const void* red;
   };
   std::vector<Bucket> buckets;
+  std::vector<int> cfg, split;  // per problem (caller's order): tile configuration and split-K
   double flops = 0.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -577,76 +732,123 @@ struct cosma_grouped_plan_s {
 
 namespace {
 
-// Tiles are numbered problem by problem, problems ordered by decreasing K
-// (the cost of each of their tiles).  The schedule hands them out
+// One bucket (one tile family): the device table, the schedule, and for the
+// split members the reduction table and their partials.  Problems are
+// ordered by decreasing tile cost; tiles are numbered problem by problem
+// (K slice, then tile row, then tile column).  The schedule hands them out
 // longest-processing-time first: each tile, in numbering order, goes to the
-// workgroup with the least work so far (K-steps plus a fixed per-tile cost
-// for the prologue and the C store), ties to the lowest logical workgroup.
-// Logical workgroup L runs as workgroup (L % (B/8)) * 8 + L / (B/8) when B,
-// the grid, is a multiple of 8: workgroups are dealt to the 8 XCDs
-// round-robin by id, so logical neighbours -- which receive the neighbouring
-// tiles of one problem in each round -- share one XCD's L2.  On 512 problems
-// with sides in [32, 2048] (256x256 tiles) this cost model puts the largest
-// workgroup load at 1.01x the mean, against 1.19x for a round-robin walk.
+// workgroup with the least work so far (tile area times its K chunks plus a
+// fixed per-tile cost for the prologue and the C store), ties to the lowest
+// logical workgroup.  Logical workgroup L runs as workgroup (L % (B/X)) * X
+// + L / (B/X) when B, the grid, is a multiple of X, the device's XCD count:
+// workgroups are dealt to the XCDs round-robin by id, so logical neighbours
+// -- which receive the neighbouring tiles of one problem in each round --
+// share one XCD's L2.  On 512 problems with sides in [32, 2048] (256x256
+// tiles) this cost model puts the largest workgroup load at 1.01x the mean,
+// against 1.19x for a round-robin walk.
+struct Member {
+  const cosma_group_problem* q;
+  int cfg, variant, split;
+};
+
 template <typename G>
-int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*>& probs_in,
-                 int blocks_per_cu, cosma_grouped_plan_s::Bucket* out) {
-  const TileDims td = kDims[mode][cfg];
-  std::vector<const cosma_group_problem*> probs(probs_in);
-  std::stable_sort(probs.begin(), probs.end(),
-                   [](const cosma_group_problem* a, const cosma_group_problem* b) { return a->k > b->k; });
+int build_bucket(int mode, int fam, std::vector<Member> mem, int blocks_per_cu, cosma_grouped_plan_s::Bucket* out) {
+  using E = G;  // accumulator type of the partials (FP32 modes: float, FP64: double)
+  const Topo topo = device_topo();
+  auto tile_cost = [&](const Member& m) -> int64_t {
+    const TileDims td = kDims[mode][m.cfg];
+    const int k = m.q->alpha == 0.0 ? 0 : m.q->k;
+    const int per = cdiv(k, m.split);
+    const int kslice = cdiv(per, td.bk) * td.bk;
+    return (int64_t)(cdiv(kslice, td.bk) + kTileOverheadSteps) * td.bm * td.bn;
+  };
+  std::stable_sort(mem.begin(), mem.end(),
+                   [&](const Member& a, const Member& b) { return tile_cost(a) > tile_cost(b); });
   std::vector<cosma::GroupEntry<G>> table;
-  table.reserve(probs.size());
-  std::vector<int64_t> cost;  // per problem: cost of each of its tiles
-  int64_t tiles = 0;
-  for (const cosma_group_problem* q : probs) {
+  std::vector<cosma::GroupReduce<G>> red;
+  std::vector<int64_t> cost;
+  std::vector<int64_t> first;  // first tile of each member
+  table.reserve(mem.size());
+  int64_t tiles = 0, ws_elems = 0, red_blocks = 0;
+  std::vector<int64_t> ws_off;
+  for (const Member& m : mem) {
+    const cosma_group_problem& q = *m.q;
+    const TileDims td = kDims[mode][m.cfg];
     cosma::GroupEntry<G> e{};
-    e.A = static_cast<const G*>(q->a);
-    e.B = static_cast<const G*>(q->b);
-    e.C = static_cast<G*>(q->c);
-    e.lda = q->lda;
-    e.ldb = q->ldb;
-    e.ldc = q->ldc;
-    e.M = q->m;
-    e.N = q->n;
-    e.K = q->k;
-    e.tiles_n = cdiv(q->n, td.bn);
-    e.tile_start = (int)tiles;
-    e.alpha = q->alpha;
-    e.beta = q->beta;
-    tiles += (int64_t)cdiv(q->m, td.bm) * e.tiles_n;
+    const bool ab = q.k > 0 && q.alpha != 0.0;
+    e.A = static_cast<const G*>(q.a);
+    e.B = static_cast<const G*>(q.b);
+    e.C = static_cast<G*>(q.c);
+    e.ws = nullptr;
+    e.lda = q.lda;
+    e.ldb = q.ldb;
+    e.ldc = q.ldc;
+    e.M = q.m;
+    e.N = q.n;
+    e.K = ab ? q.k : 0;  // alpha == 0 or K == 0: the tiles only scale C by beta
+    e.variant = m.variant;
+    e.tiles_m = cdiv(q.m, td.bm);
+    e.tiles_n = cdiv(q.n, td.bn);
+    e.split = ab ? m.split : 1;
+    const int per = cdiv(e.K, e.split);
+    e.kslice = cdiv(per, td.bk) * td.bk;
+    e.vec_a = ab && ((uintptr_t)q.a % 16 == 0) && (q.lda % (16 / (int)sizeof(G)) == 0);
+    e.vec_b = ab && ((uintptr_t)q.b % 16 == 0) && (q.ldb % (16 / (int)sizeof(G)) == 0);
+    e.alpha = ab ? q.alpha : 0.0;
+    e.beta = q.beta;
+    first.push_back(tiles);
+    tiles += (int64_t)e.tiles_m * e.tiles_n * e.split;
     if (tiles > 0x3fffffff) return COSMA_STATUS_INVALID_VALUE;  // two schedule ints per tile
+    ws_off.push_back(-1);
+    if (e.split > 1) {
+      ws_off.back() = ws_elems;
+      const int64_t elems = (int64_t)e.split * q.m * q.n;
+      ws_elems += (elems + 63) / 64 * 64;  // members 256-byte aligned
+      cosma::GroupReduce<G> r{};
+      r.C = e.C;
+      r.ldc = q.ldc;
+      r.M = q.m;
+      r.N = q.n;
+      r.S = e.split;
+      r.block0 = (int)red_blocks;
+      r.alpha = e.alpha;
+      r.beta = e.beta;
+      red_blocks += ((int64_t)q.m * q.n + 255) / 256;
+      if (red_blocks > 0x7fffffff) return COSMA_STATUS_INVALID_VALUE;
+      red.push_back(r);
+    }
     table.push_back(e);
-    cost.push_back(cdiv(q->k, td.bk) + kTileOverheadSteps);
+    cost.push_back(tile_cost(m));
   }
-  out->cfg = cfg;
+  out->fam = fam;
   out->count = (int)table.size();
   out->tiles = (int)tiles;
-  const int occ = blocks_per_cu > 0 ? blocks_per_cu : cosma::grouped_occupancy(mode, cfg);
-  int64_t blocks64 = (int64_t)occ * kCUs;
+  out->nred = (int)red.size();
+  out->red_blocks = (int)red_blocks;
+  const int occ = blocks_per_cu > 0 ? blocks_per_cu : cosma::grouped_occupancy(mode, fam);
+  int64_t blocks64 = (int64_t)occ * topo.cus;
   if (blocks64 > tiles) blocks64 = tiles;
   const int B = (int)(blocks64 > 0 ? blocks64 : 1);
   out->blocks = B;
 
-  // Longest-processing-time-first schedule (see above).
   std::vector<std::vector<std::pair<int, int>>> lists(B);  // (problem, tile within it)
-  using Slot = std::pair<int64_t, int>;  // (load, logical workgroup)
+  using Slot = std::pair<int64_t, int>;                    // (load, logical workgroup)
   std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
   for (int l = 0; l < B; ++l) heap.push(Slot{0, l});
   for (size_t i = 0; i < table.size(); ++i) {
-    const int64_t first = table[i].tile_start;
-    const int64_t last = i + 1 < table.size() ? table[i + 1].tile_start : tiles;
-    for (int64_t t = first; t < last; ++t) {
+    const int64_t n_i = (int64_t)table[i].tiles_m * table[i].tiles_n * table[i].split;
+    for (int64_t t = 0; t < n_i; ++t) {
       Slot s = heap.top();
       heap.pop();
-      lists[s.second].push_back({(int)i, (int)(t - first)});
+      lists[s.second].push_back({(int)i, (int)t});
       heap.push(Slot{s.first + cost[i], s.second});
     }
   }
-  const int per_xcd = B % 8 == 0 ? B / 8 : 0;
+  const int X = topo.xcds > 1 ? topo.xcds : 1;
+  const int per_xcd = B % X == 0 ? B / X : 0;
   std::vector<int> sched((size_t)B + 1 + 2 * (size_t)tiles);
   std::vector<int> of_phys(B);  // logical workgroup run by physical id
-  for (int l = 0; l < B; ++l) of_phys[per_xcd ? (l % per_xcd) * 8 + l / per_xcd : l] = l;
+  for (int l = 0; l < B; ++l) of_phys[per_xcd ? (l % per_xcd) * X + l / per_xcd : l] = l;
   int pos = 0;
   for (int w = 0; w < B; ++w) {
     sched[w] = pos;
@@ -658,19 +860,32 @@ int build_bucket(int mode, int cfg, const std::vector<const cosma_group_problem*
   }
   sched[B] = pos;
 
-  const size_t tbytes = (table.size() * sizeof(cosma::GroupEntry<G>) + 255) / 256 * 256;
-  const size_t sbytes = sched.size() * sizeof(int);
-  out->table = nullptr;
-  out->sched = nullptr;
-  hipError_t e = hipMalloc(&out->table, tbytes + sbytes);
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t tbytes = up(table.size() * sizeof(cosma::GroupEntry<G>));
+  const size_t sbytes = up(sched.size() * sizeof(int));
+  const size_t rbytes = up(red.size() * sizeof(cosma::GroupReduce<G>));
+  const size_t wbytes = (size_t)ws_elems * sizeof(E);
+  out->mem = nullptr;
+  hipError_t e = hipMalloc(&out->mem, tbytes + sbytes + rbytes + wbytes);
   if (e != hipSuccess) {
-    out->table = nullptr;
+    out->mem = nullptr;
     return (int)e;
   }
-  out->sched = reinterpret_cast<const int*>(static_cast<char*>(out->table) + tbytes);
-  e = hipMemcpy(out->table, table.data(), table.size() * sizeof(cosma::GroupEntry<G>), hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(const_cast<int*>(out->sched), sched.data(), sbytes, hipMemcpyHostToDevice);
+  char* base = static_cast<char*>(out->mem);
+  out->table = base;
+  out->sched = reinterpret_cast<const int*>(base + tbytes);
+  out->red = base + tbytes + sbytes;
+  E* ws = reinterpret_cast<E*>(base + tbytes + sbytes + rbytes);
+  for (size_t i = 0, r = 0; i < table.size(); ++i) {
+    if (ws_off[i] < 0) continue;
+    table[i].ws = ws + ws_off[i];
+    red[r++].ws = ws + ws_off[i];
+  }
+  e = hipMemcpy(base, table.data(), table.size() * sizeof(cosma::GroupEntry<G>), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + tbytes, sched.data(), sched.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !red.empty())
+    e = hipMemcpy(base + tbytes + sbytes, red.data(), red.size() * sizeof(cosma::GroupReduce<G>),
+                  hipMemcpyHostToDevice);
   return (int)e;
 }
 
@@ -679,65 +894,119 @@ void free_plan(cosma_grouped_plan_s* p) {
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
   for (auto& b : p->buckets)
-    if (b.table) hipFree(b.table);
+    if (b.mem) hipFree(b.mem);
   delete p;
+}
+
+// One bucket: the persistent launch, then the reduction of its split members.
+int launch_bucket(int mode, const cosma_grouped_plan_s::Bucket& b, hipStream_t s) {
+  if (b.tiles == 0) return 0;
+  hipError_t e = cosma::run_grouped(mode, b.fam, b.table, b.sched, b.blocks, s);
+  if (e == hipSuccess && b.nred > 0) e = cosma::run_grouped_reduce(mode, b.red, b.nred, b.red_blocks, s);
+  return (int)e;
 }
 
 int launch_plan(cosma_grouped_plan_s* p, hipStream_t s) {
   for (auto& b : p->buckets) {
-    if (b.tiles == 0) continue;
-    hipError_t e = cosma::run_grouped(p->mode, b.cfg, b.table, b.count, b.sched, b.blocks, s);
-    if (e != hipSuccess) return (int)e;
+    const int st = launch_bucket(p->mode, b, s);
+    if (st != 0) return st;
   }
   return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* problems,
-                              const int* tile_cfgs, int blocks_per_cu, cosma_grouped_plan* plan) {
-  if (!plan) return COSMA_STATUS_INVALID_VALUE;
-  *plan = nullptr;
-  if (mode < 0 || mode >= kNumModes || count < 0 || (count > 0 && !problems))
-    return COSMA_STATUS_INVALID_VALUE;
-  std::vector<std::vector<const cosma_group_problem*>> by_cfg(kNumCfg[mode]);
-  const int default_cfg = group_default_cfg(mode, count, problems);
+// Plan creation (cosma_grouped_plan_create_ex without the argument checks on
+// the plan pointer).
+int create_plan(int mode, int count, const cosma_group_problem* problems, const int* tile_cfgs, const int* splits,
+                int blocks_per_cu, cosma_grouped_plan_s** out) {
+  *out = nullptr;
+  if (mode < 0 || mode >= kNumModes || count < 0 || (count > 0 && !problems)) return COSMA_STATUS_INVALID_VALUE;
+  std::vector<int> dcfg;
+  group_default_cfgs(mode, count, problems, dcfg);
+  std::vector<int> cfg(count, -1), split(count, 1);
+  std::vector<const cosma_group_problem*> live;
+  std::vector<int> live_idx;
   double flops = 0.0;
   for (int i = 0; i < count; ++i) {
     const cosma_group_problem& q = problems[i];
-    cosma_gemm_desc d = make_desc(mode, q.m, q.n, q.k, q.alpha, q.a, (int)0, q.b, 0, q.beta, q.c,
-                                  0, nullptr);
+    cosma_gemm_desc d = make_desc(mode, q.m, q.n, q.k, q.alpha, q.a, 0, q.b, 0, q.beta, q.c, 0, nullptr);
     d.lda = q.lda;
     d.ldb = q.ldb;
     d.ldc = q.ldc;
     int st = validate(&d);
     if (st != 0) return st;
+    if (splits && splits[i] > kMaxSplit) return COSMA_STATUS_INVALID_VALUE;
+    int c = (tile_cfgs && tile_cfgs[i] >= 0) ? tile_cfgs[i] : dcfg[i];
+    if (c >= kNumCfg[mode]) return COSMA_STATUS_INVALID_VALUE;
+    if (is_planar(mode, c)) return COSMA_STATUS_NOT_SUPPORTED;
+    cfg[i] = c;
     if (q.m == 0 || q.n == 0) continue;
-    if (q.k > 0 && q.alpha != 0.0 && ((uintptr_t)q.c == 0)) return COSMA_STATUS_INVALID_VALUE;
-    int cfg = (tile_cfgs && tile_cfgs[i] >= 0) ? tile_cfgs[i] : default_cfg;
-    if (cfg >= kNumCfg[mode]) return COSMA_STATUS_INVALID_VALUE;
-    if (is_planar(mode, cfg)) return COSMA_STATUS_NOT_SUPPORTED;
-    by_cfg[cfg].push_back(&q);
+    live.push_back(&q);
+    live_idx.push_back(i);
     flops += 2.0 * q.m * q.n * (double)q.k;
+  }
+  // Default splits, from the whole batch's load per persistent workgroup.
+  {
+    std::vector<int> lcfg(live.size());
+    for (size_t j = 0; j < live.size(); ++j) lcfg[j] = cfg[live_idx[j]];
+    int fam = 0, var = 0;
+    family_of(mode, live.empty() ? 0 : lcfg[0], &fam, &var);
+    const int slots = (blocks_per_cu > 0 ? blocks_per_cu : cosma::grouped_occupancy(mode, fam)) * device_topo().cus;
+    std::vector<int> dsplit;
+    group_default_splits(mode, live, lcfg, slots, dsplit);
+    for (size_t j = 0; j < live.size(); ++j) {
+      const int i = live_idx[j];
+      const cosma_group_problem& q = problems[i];
+      int s = (splits && splits[i] > 0) ? splits[i] : dsplit[j];
+      if (s > q.k) s = q.k > 0 ? q.k : 1;
+      split[i] = std::max(1, s);
+    }
+  }
+  int nf = 0;
+  cosma::group_families(mode, &nf);
+  std::vector<std::vector<Member>> by_fam(nf);
+  for (size_t j = 0; j < live.size(); ++j) {
+    const int i = live_idx[j];
+    int fam = 0, var = 0;
+    if (!family_of(mode, cfg[i], &fam, &var)) return COSMA_STATUS_NOT_SUPPORTED;
+    by_fam[fam].push_back(Member{live[j], cfg[i], var, split[i]});
   }
   cosma_grouped_plan_s* p = new cosma_grouped_plan_s();
   p->mode = mode;
   p->flops = flops;
-  for (int cfg = 0; cfg < kNumCfg[mode]; ++cfg) {
-    if (by_cfg[cfg].empty()) continue;
+  p->cfg = cfg;
+  p->split = split;
+  for (int f = 0; f < nf; ++f) {
+    if (by_fam[f].empty()) continue;
     cosma_grouped_plan_s::Bucket b{};
-    int st = mode == COSMA_MODE_FP64 ? build_bucket<double>(mode, cfg, by_cfg[cfg], blocks_per_cu, &b)
-                                     : build_bucket<float>(mode, cfg, by_cfg[cfg], blocks_per_cu, &b);
+    int st = mode == COSMA_MODE_FP64 ? build_bucket<double>(mode, f, by_fam[f], blocks_per_cu, &b)
+                                     : build_bucket<float>(mode, f, by_fam[f], blocks_per_cu, &b);
     p->buckets.push_back(b);
     if (st != 0) {
       free_plan(p);
       return st;
     }
   }
-  *plan = p;
+  *out = p;
   return 0;
+}
+
+// Size bucket of a problem for the grouped autotuner: its default tile
+// configuration and a K class (K <= 256, <= 1024, above).
+int size_bucket(int cfg, int k) { return cfg * 3 + (k <= 256 ? 0 : (k <= 1024 ? 1 : 2)); }
+
+}  // namespace
+
+extern "C" {
+
+int cosma_grouped_plan_create_ex(int mode, int count, const cosma_group_problem* problems, const int* tile_cfgs,
+                                 const int* splits, int blocks_per_cu, cosma_grouped_plan* plan) {
+  if (!plan) return COSMA_STATUS_INVALID_VALUE;
+  return create_plan(mode, count, problems, tile_cfgs, splits, blocks_per_cu, plan);
+}
+
+int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem* problems, const int* tile_cfgs,
+                              int blocks_per_cu, cosma_grouped_plan* plan) {
+  return cosma_grouped_plan_create_ex(mode, count, problems, tile_cfgs, nullptr, blocks_per_cu, plan);
 }
 
 int cosma_grouped_plan_run(cosma_grouped_plan plan, void* stream) {
@@ -749,9 +1018,10 @@ int cosma_grouped_plan_run_graph(cosma_grouped_plan plan, void* stream) {
   if (!plan) return COSMA_STATUS_INVALID_VALUE;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (!plan->exec) {
-    // Captured as one graph whose bucket launches are independent branches
-    // (fork/join through events), so a later bucket's workgroups start on CUs
-    // that an earlier bucket's tail leaves idle.
+    // Captured as one graph whose buckets are independent branches (fork /
+    // join through events), so a later bucket's workgroups start on CUs that
+    // an earlier bucket's tail leaves idle; each branch is the persistent
+    // launch followed by the reduction of its split members.
     hipStream_t cs = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
     if (e != hipSuccess) return (int)e;
@@ -770,9 +1040,7 @@ int cosma_grouped_plan_run_graph(cosma_grouped_plan plan, void* stream) {
     for (int b = 0; b < nb && st == 0; ++b) {
       hipStream_t bs = b == 0 ? cs : side[b - 1];
       if (b > 0) st = (int)hipStreamWaitEvent(bs, ev[0], 0);
-      const auto& k = plan->buckets[b];
-      if (st == 0 && k.tiles > 0)
-        st = (int)cosma::run_grouped(plan->mode, k.cfg, k.table, k.count, k.sched, k.blocks, bs);
+      if (st == 0) st = launch_bucket(plan->mode, plan->buckets[b], bs);
       if (st == 0 && b > 0) st = (int)hipEventRecord(ev[b], bs);
     }
     for (int b = 1; b < nb && st == 0; ++b) st = (int)hipStreamWaitEvent(cs, ev[b], 0);  // join
@@ -810,14 +1078,22 @@ int cosma_grouped_plan_info(cosma_grouped_plan plan, int* buckets, int* total_ti
   return 0;
 }
 
-int cosma_grouped_plan_bucket(cosma_grouped_plan plan, int b, int* tile_cfg, int* count, int* tiles,
-                              int* blocks) {
+int cosma_grouped_plan_bucket(cosma_grouped_plan plan, int b, int* tile_cfg, int* count, int* tiles, int* blocks) {
   if (!plan || b < 0 || b >= (int)plan->buckets.size()) return COSMA_STATUS_INVALID_VALUE;
   const auto& k = plan->buckets[b];
-  if (tile_cfg) *tile_cfg = k.cfg;
+  int nf = 0;
+  const cosma::GroupFamily* f = cosma::group_families(plan->mode, &nf);
+  if (tile_cfg) *tile_cfg = f[k.fam].cfgs[0];
   if (count) *count = k.count;
   if (tiles) *tiles = k.tiles;
   if (blocks) *blocks = k.blocks;
+  return 0;
+}
+
+int cosma_grouped_plan_choice(cosma_grouped_plan plan, int i, int* tile_cfg, int* split_k) {
+  if (!plan || i < 0 || i >= (int)plan->cfg.size()) return COSMA_STATUS_INVALID_VALUE;
+  if (tile_cfg) *tile_cfg = plan->cfg[i];
+  if (split_k) *split_k = plan->split[i];
   return 0;
 }
 
@@ -825,6 +1101,138 @@ int cosma_grouped_plan_destroy(cosma_grouped_plan plan) {
   free_plan(plan);
   return 0;
 }
+
+int cosma_group_families(int mode, int fam, int* cfgs4) {
+  if (mode < 0 || mode >= kNumModes || !cfgs4) return COSMA_STATUS_INVALID_VALUE;
+  int nf = 0;
+  const cosma::GroupFamily* f = cosma::group_families(mode, &nf);
+  if (fam < 0) return nf;
+  if (fam >= nf) return COSMA_STATUS_INVALID_VALUE;
+  for (int v = 0; v < 4; ++v) cfgs4[v] = v < f[fam].ncfg ? f[fam].cfgs[v] : -1;
+  return f[fam].ncfg;
+}
+
+int cosma_grouped_autotune(int mode, int count, const cosma_group_problem* problems, int warmup, int reps,
+                           int* tile_cfgs, int* splits, cosma_group_tune_result* results, int max_results,
+                           int* n_results) {
+  if (!n_results || !tile_cfgs || !splits || (max_results > 0 && !results) || warmup < 0 || reps < 1)
+    return COSMA_STATUS_INVALID_VALUE;
+  *n_results = 0;
+  // The default plan's choices are the starting point.
+  cosma_grouped_plan_s* p0 = nullptr;
+  int st = create_plan(mode, count, problems, nullptr, nullptr, 0, &p0);
+  if (st != 0) return st;
+  std::vector<int> cfg = p0->cfg, split = p0->split, base_cfg = p0->cfg;
+  free_plan(p0);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    if (e0) hipEventDestroy(e0);
+    return (int)he;
+  }
+  std::vector<cosma_group_tune_result> all;
+  // Time per pass of a plan: graph replays on the null stream after warmup
+  // replays.
+  auto time_one = [&](cosma_grouped_plan_s* p, double* ms) -> int {
+    int rc = 0;
+    for (int i = 0; i < warmup && rc == 0; ++i) rc = cosma_grouped_plan_run_graph(p, nullptr);
+    if (rc == 0) rc = (int)hipEventRecord(e0, nullptr);
+    for (int i = 0; i < reps && rc == 0; ++i) rc = cosma_grouped_plan_run_graph(p, nullptr);
+    if (rc == 0) rc = (int)hipEventRecord(e1, nullptr);
+    if (rc == 0) rc = (int)hipEventSynchronize(e1);
+    float t = 0.f;
+    if (rc == 0) rc = (int)hipEventElapsedTime(&t, e0, e1);
+    *ms = rc == 0 ? (double)t / reps : 0.0;
+    return rc;
+  };
+  auto record = [&](int bucket, int c, int s, int rc, double ms) {
+    cosma_group_tune_result r{};
+    r.bucket = bucket;
+    r.tile_cfg = c;
+    r.split_k = s;
+    r.status = rc;
+    r.ms = ms;
+    all.push_back(r);
+  };
+  // The incumbent plan (the choices kept so far).  A candidate is timed in
+  // pairs with it -- incumbent, candidate, candidate, incumbent -- and kept
+  // only when it is at least 1% faster on both halves, so clock drift over
+  // the search (the GPU's clock settles under load) cannot pass for a gain.
+  cosma_grouped_plan_s* inc = nullptr;
+  st = create_plan(mode, count, problems, cfg.data(), split.data(), 0, &inc);
+  double warm = 0.0;
+  for (int i = 0; i < 3 && st == 0; ++i) st = time_one(inc, &warm);  // settle the clocks
+  record(-1, -1, 0, st, warm);
+  if (st != 0) {
+    free_plan(inc);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+  }
+  // Size buckets in decreasing order of their flops.
+  std::map<int, double> bflops;
+  for (int i = 0; i < count; ++i)
+    if (problems[i].m > 0 && problems[i].n > 0)
+      bflops[size_bucket(base_cfg[i], problems[i].k)] += (double)problems[i].m * problems[i].n * problems[i].k;
+  std::vector<std::pair<double, int>> order;
+  for (auto& kv : bflops) order.push_back({-kv.second, kv.first});
+  std::sort(order.begin(), order.end());
+  // Candidates: every configuration of the mode's first family, the
+  // bucket's default, and K split 1, 2 or 4 ways (slices at least two K
+  // chunks deep).
+  int nf = 0;
+  const cosma::GroupFamily& f0 = cosma::group_families(mode, &nf)[0];
+  for (const auto& ob : order) {
+    const int bk = ob.second;
+    std::vector<int> cands(f0.cfgs, f0.cfgs + f0.ncfg);
+    const int dflt = bk / 3;
+    if (std::find(cands.begin(), cands.end(), dflt) == cands.end()) cands.push_back(dflt);
+    for (int c : cands) {
+      for (int s : {1, 2, 4}) {
+        std::vector<int> keep_c = cfg, keep_s = split;
+        bool changed = false, ok = true;
+        for (int i = 0; i < count; ++i) {
+          if (problems[i].m <= 0 || problems[i].n <= 0 || size_bucket(base_cfg[i], problems[i].k) != bk) continue;
+          const int si = (s > 1 && problems[i].k >= 2 * s * kDims[mode][c].bk) ? s : 1;
+          if (si != s) ok = false;
+          if (cfg[i] != c || split[i] != si) changed = true;
+          cfg[i] = c;
+          split[i] = si;
+        }
+        cosma_grouped_plan_s* cand = nullptr;
+        int rc = (changed && ok) ? create_plan(mode, count, problems, cfg.data(), split.data(), 0, &cand) : -1;
+        double ti0 = 0, tc0 = 0, tc1 = 0, ti1 = 0;
+        if (rc == 0) rc = time_one(inc, &ti0);
+        if (rc == 0) rc = time_one(cand, &tc0);
+        if (rc == 0) rc = time_one(cand, &tc1);
+        if (rc == 0) rc = time_one(inc, &ti1);
+        if (changed && ok) record(bk, c, s, rc, 0.5 * (tc0 + tc1));
+        if (rc == 0 && tc0 < 0.99 * ti0 && tc1 < 0.99 * ti1) {
+          free_plan(inc);
+          inc = cand;
+        } else {
+          if (cand) free_plan(cand);
+          cfg = keep_c;
+          split = keep_s;
+        }
+      }
+    }
+  }
+  free_plan(inc);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::copy(cfg.begin(), cfg.end(), tile_cfgs);
+  std::copy(split.begin(), split.end(), splits);
+  const int n = std::min((int)all.size(), max_results);
+  for (int i = 0; i < n; ++i) results[i] = all[i];
+  *n_results = n;
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 
 void cosma_gemm_desc_init(cosma_gemm_desc* d) {
@@ -881,6 +1289,7 @@ int cosma_split_operand(int mode, int kmajor, int rows, int k, const float* X, i
 }
 
 int cosma_workspace_set(void* ptr, size_t bytes) {
+  if ((uintptr_t)ptr % 16) return COSMA_STATUS_MISALIGNED;
   std::lock_guard<std::mutex> lk(g_mu);
   g_user_ws = bytes ? ptr : nullptr;
   g_user_ws_bytes = ptr ? bytes : 0;
@@ -995,13 +1404,26 @@ int cosma_tile_config(int mode, int cfg, int* out8) {
   return 0;
 }
 
-int cosma_heuristic(int mode, int m, int n, int k, int* tile_cfg, int* split_k) {
-  if (mode < 0 || mode >= kNumModes || m < 0 || n < 0 || k < 0 || !tile_cfg || !split_k)
+int cosma_heuristic_topo(int mode, int m, int n, int k, int trans_a, int cus, int* tile_cfg, int* split_k) {
+  if (mode < 0 || mode >= kNumModes || m < 0 || n < 0 || k < 0 || (trans_a != 0 && trans_a != 1) || cus < 1 ||
+      !tile_cfg || !split_k)
     return COSMA_STATUS_INVALID_VALUE;
-  Choice ch = heuristic(mode, m, n, k);
+  const Choice ch = heuristic(mode, m, n, k, trans_a, cus);
   *tile_cfg = ch.cfg;
   *split_k = ch.split;
   return 0;
+}
+
+int cosma_heuristic(int mode, int m, int n, int k, int* tile_cfg, int* split_k) {
+  return cosma_heuristic_topo(mode, m, n, k, 0, device_topo().cus, tile_cfg, split_k);
+}
+
+int cosma_device_topology(int* cus, int* xcds, int* clock_khz) {
+  const Topo t = device_topo();
+  if (cus) *cus = t.cus;
+  if (xcds) *xcds = t.xcds;
+  if (clock_khz) *clock_khz = t.clock_khz;
+  return t.queried ? 0 : COSMA_STATUS_NOT_SUPPORTED;
 }
 
 int cosma_error_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
@@ -1059,7 +1481,7 @@ int cosma_tune_set(int mode, int m, int n, int k, int trans_a, int trans_b, int 
                    int split_k, int group_m) {
   if (mode < 0 || mode >= kNumModes || m < 0 || n < 0 || k < 0 || (trans_a != 0 && trans_a != 1) ||
       (trans_b != 0 && trans_b != 1) || tile_cfg < 0 || tile_cfg >= kNumCfg[mode] || split_k < 1 ||
-      split_k > kMaxSplit || group_m < 0)
+      split_k > kMaxSplit || group_m < 0 || group_m > kMaxGroupM)
     return COSMA_STATUS_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[TuneKey{mode, m, n, k, trans_a, trans_b}] = TuneVal{tile_cfg, split_k, group_m};
@@ -1083,8 +1505,8 @@ int cosma_tune_clear(void) {
   return 0;
 }
 
-int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_result* results,
-                   int max_results, int* n_results, int* best) {
+int cosma_autotune_ex(const cosma_gemm_desc* d, int warmup, int reps, size_t flush_bytes,
+                      cosma_tune_result* results, int max_results, int* n_results, int* best) {
   int st = validate(d);
   if (st != 0) return st;
   if (!n_results || !best || (max_results > 0 && !results) || warmup < 0 || reps < 1)
@@ -1092,13 +1514,21 @@ int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_re
   *n_results = 0;
   *best = -1;
   if (d->m == 0 || d->n == 0 || d->k == 0 || d->alpha == 0.0) return 0;
-  const double peak[kNumModes] = {157.3, 78.6, 2516.6 / 3.0, 2516.6 / 4.0};
+  const Topo topo = device_topo();
+  // Dense MFMA peak of the mode from the device: CUs x clock x per-CU rate
+  // (FP32 256, FP64 128, BF16 4096 flops per clock; BF16X3 / BF16X4 count
+  // useful flops, a third / a quarter of the bf16 work).
+  const double clk = (topo.clock_khz > 0 ? topo.clock_khz : 2400000) * 1e3;
+  const double per_cu[kNumModes] = {256.0, 128.0, 4096.0 / 3.0, 4096.0 / 4.0};
+  const double peak_tf = per_cu[d->mode] * topo.cus * clk / 1e12;
   hipStream_t s = static_cast<hipStream_t>(d->stream);
+  void* flush = nullptr;
+  if (flush_bytes > 0 && (st = flush_buffer(flush_bytes, s, &flush)) != 0) return st;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t he = hipEventCreate(&e0);
   if (he == hipSuccess) he = hipEventCreate(&e1);
   if (he != hipSuccess) {
-    if (e0) hipEventDestroy(e0);
+    if (e0) (void)hipEventDestroy(e0);
     return (int)he;
   }
   const double flops = 2.0 * d->m * (double)d->n * d->k;
@@ -1123,17 +1553,25 @@ int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_re
     r.waves_per_simd = ti ? r.occupancy * (ti->threads / 64) / 4 : 0;
     int status = 0;
     for (int i = 0; i < warmup && status == 0; ++i) status = run(&c);
-    if (status == 0) status = (int)hipEventRecord(e0, s);
-    for (int i = 0; i < reps && status == 0; ++i) status = run(&c);
-    if (status == 0) status = (int)hipEventRecord(e1, s);
-    if (status == 0) status = (int)hipEventSynchronize(e1);
-    float ms = 0.f;
-    if (status == 0) status = (int)hipEventElapsedTime(&ms, e0, e1);
+    // Each timed call: the cache flush (when asked), then the call between
+    // an event pair, so candidates are compared with cold caches as the
+    // benchmark protocol runs them.
+    double total = 0.0;
+    for (int i = 0; i < reps && status == 0; ++i) {
+      if (flush) status = (int)cosma::run_flush(flush, flush_bytes, s);
+      if (status == 0) status = (int)hipEventRecord(e0, s);
+      if (status == 0) status = run(&c);
+      if (status == 0) status = (int)hipEventRecord(e1, s);
+      if (status == 0) status = (int)hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (status == 0) status = (int)hipEventElapsedTime(&ms, e0, e1);
+      total += ms;
+    }
     r.status = status;
     if (status == 0) {
-      r.ms = ms / reps;
+      r.ms = total / reps;
       r.tflops = r.ms > 0 ? flops / (r.ms * 1e-3) / 1e12 : 0.0;
-      r.mfma_util = r.tflops / peak[d->mode];
+      r.achieved_frac = r.tflops / peak_tf;
     }
     all.push_back(r);
   };
@@ -1146,7 +1584,7 @@ int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_re
     const int tiles = tiles_of(d->mode, cfg, d->m, d->n);
     for (int split = 1; split <= 1024; split *= 2) {
       if (d->split_k > 0 && split != d->split_k) continue;
-      if (split > 1 && (split * 2 * td.bk > d->k || (int64_t)tiles * split > 16 * kCUs)) continue;
+      if (split > 1 && (split * 2 * td.bk > d->k || (int64_t)tiles * split > 16 * (int64_t)topo.cus)) continue;
       try_one(cfg, split, gm0);
     }
     if (d->split_k > 1024) try_one(cfg, d->split_k, gm0);
@@ -1165,39 +1603,33 @@ int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_re
     if (cdiv(d->m, kDims[d->mode][cfg].bm) > 8)
       for (int gm : {2, 4, 16, 32}) try_one(cfg, split, gm);
   }
-  he = hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  *best = best_of();
-  const int n = (int)all.size() < max_results ? (int)all.size() : max_results;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // Fastest first (failed candidates last, in trial order), so the best
+  // candidate is reported even when max_results is smaller than the count.
+  std::stable_sort(all.begin(), all.end(), [](const cosma_tune_result& x, const cosma_tune_result& y) {
+    if ((x.status == 0) != (y.status == 0)) return x.status == 0;
+    return x.status == 0 && x.ms < y.ms;
+  });
+  const int n = std::min((int)all.size(), std::max(max_results, 0));
   for (int i = 0; i < n; ++i) results[i] = all[i];
   *n_results = n;
-  if (*best >= n) *best = -1;
+  *best = (!all.empty() && all[0].status == 0) ? 0 : -1;
   return 0;
 }
 
-int cosma_benchmark(const cosma_gemm_desc* d, int warmup, int runs, size_t flush_bytes,
-                    float* ms_out) {
+int cosma_autotune(const cosma_gemm_desc* d, int warmup, int reps, cosma_tune_result* results, int max_results,
+                   int* n_results, int* best) {
+  return cosma_autotune_ex(d, warmup, reps, kDefaultFlushBytes, results, max_results, n_results, best);
+}
+
+int cosma_benchmark(const cosma_gemm_desc* d, int warmup, int runs, size_t flush_bytes, float* ms_out) {
   int st = validate(d);
   if (st != 0) return st;
   if (warmup < 0 || runs < 0 || (runs > 0 && !ms_out)) return COSMA_STATUS_INVALID_VALUE;
   hipStream_t s = static_cast<hipStream_t>(d->stream);
   void* flush = nullptr;
-  if (flush_bytes > 0) {
-    static std::mutex fmu;
-    static void* fbuf = nullptr;
-    static size_t fbytes = 0;
-    std::lock_guard<std::mutex> lk(fmu);
-    if (fbytes < flush_bytes) {
-      if (capturing(s)) return COSMA_STATUS_WORKSPACE;
-      if (fbuf) hipFree(fbuf);
-      fbuf = nullptr;
-      fbytes = 0;
-      hipError_t e = hipMalloc(&fbuf, flush_bytes);
-      if (e != hipSuccess) return (int)e;
-      fbytes = flush_bytes;
-    }
-    flush = fbuf;
-  }
+  if (flush_bytes > 0 && (st = flush_buffer(flush_bytes, s, &flush)) != 0) return st;
   for (int i = 0; i < warmup; ++i) {
     st = run(d);
     if (st) return st;
@@ -1207,7 +1639,7 @@ int cosma_benchmark(const cosma_gemm_desc* d, int warmup, int runs, size_t flush
   hipEvent_t e0, e1;
   if ((e = hipEventCreate(&e0)) != hipSuccess) return (int)e;
   if ((e = hipEventCreate(&e1)) != hipSuccess) {
-    hipEventDestroy(e0);
+    (void)hipEventDestroy(e0);
     return (int)e;
   }
   for (int i = 0; i < runs && st == 0; ++i) {
@@ -1218,8 +1650,8 @@ int cosma_benchmark(const cosma_gemm_desc* d, int warmup, int runs, size_t flush
     if (st == 0) st = (int)hipEventSynchronize(e1);
     if (st == 0) st = (int)hipEventElapsedTime(&ms_out[i], e0, e1);
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return st;
 }
 

@@ -1,7 +1,6 @@
 // cosma_aux.hip -- split-K reduction, beta scaling, FP64 residual statistics
 // and the tile-configuration table.
-#include "cosma_internal.hpp"
-#include "cosma_kernels.hpp"
+#include "cosma_launch.hpp"
 
 namespace cosma {
 
@@ -160,80 +159,71 @@ __global__ void __launch_bounds__(256)
 }
 
 
-extern const TileInfo kF32Tiles[];
-extern const int kF32NumTiles;
-extern const TileInfo kF64Tiles[];
-extern const int kF64NumTiles;
-extern const TileInfo kBF16x3Tiles[];
-extern const int kBF16x3NumTiles;
-extern const TileInfo kBF16x4Tiles[];
-extern const int kBF16x4NumTiles;
+const ModeOps* ops_f32();
+const ModeOps* ops_f64();
+const ModeOps* ops_bf16x3();
+const ModeOps* ops_bf16x4();
 
-hipError_t run_grouped_f32(int cfg, const void* table, int count, const int* sched, int blocks, hipStream_t s);
-hipError_t run_grouped_f64(int cfg, const void* table, int count, const int* sched, int blocks, hipStream_t s);
-hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, const int* sched, int blocks,
-                              hipStream_t s);
-int grouped_occupancy_f32(int cfg);
-int grouped_occupancy_f64(int cfg);
-int grouped_occupancy_bf16x3(int cfg);
-hipError_t run_grouped_bf16x4(int cfg, const void* table, int count, const int* sched, int blocks,
-                              hipStream_t s);
-int grouped_occupancy_bf16x4(int cfg);
-
-hipError_t run_grouped(int mode, int cfg, const void* table, int count, const int* sched, int blocks,
-                       hipStream_t s) {
+static const ModeOps* ops(int mode) {
   switch (mode) {
-    case kF32: return run_grouped_f32(cfg, table, count, sched, blocks, s);
-    case kF64: return run_grouped_f64(cfg, table, count, sched, blocks, s);
-    case kBF16x3: return run_grouped_bf16x3(cfg, table, count, sched, blocks, s);
-    case kBF16x4: return run_grouped_bf16x4(cfg, table, count, sched, blocks, s);
-    default: return hipErrorInvalidValue;
+    case kF32: return ops_f32();
+    case kF64: return ops_f64();
+    case kBF16x3: return ops_bf16x3();
+    case kBF16x4: return ops_bf16x4();
+    default: return nullptr;
   }
 }
 
-int grouped_occupancy(int mode, int cfg) {
-  switch (mode) {
-    case kF32: return grouped_occupancy_f32(cfg);
-    case kF64: return grouped_occupancy_f64(cfg);
-    case kBF16x3: return grouped_occupancy_bf16x3(cfg);
-    case kBF16x4: return grouped_occupancy_bf16x4(cfg);
-    default: return 1;
-  }
-}
-
-int gemm_occupancy_f32(int cfg, int ta, int tb, bool edge);
-int gemm_occupancy_f64(int cfg, int ta, int tb, bool edge);
-int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge);
-int gemm_occupancy_bf16x4(int cfg, int ta, int tb, bool edge);
-
-int gemm_occupancy(int mode, int cfg, int ta, int tb, bool edge) {
-  switch (mode) {
-    case kF32: return gemm_occupancy_f32(cfg, ta, tb, edge);
-    case kF64: return gemm_occupancy_f64(cfg, ta, tb, edge);
-    case kBF16x3: return gemm_occupancy_bf16x3(cfg, ta, tb, edge);
-    case kBF16x4: return gemm_occupancy_bf16x4(cfg, ta, tb, edge);
-    default: return 0;
-  }
+hipError_t run_gemm(const Problem& p) {
+  const ModeOps* o = ops(p.mode);
+  if (!o || p.cfg < 0 || p.cfg >= o->ncfg) return hipErrorInvalidValue;
+  return o->run(p.cfg, p);
 }
 
 int num_tile_cfgs(int mode) {
-  switch (mode) {
-    case kF32: return kF32NumTiles;
-    case kF64: return kF64NumTiles;
-    case kBF16x3: return kBF16x3NumTiles;
-    case kBF16x4: return kBF16x4NumTiles;
-    default: return 0;
-  }
+  const ModeOps* o = ops(mode);
+  return o ? o->ncfg : 0;
 }
 
 const TileInfo* tile_info(int mode, int cfg) {
-  if (cfg < 0 || cfg >= num_tile_cfgs(mode)) return nullptr;
-  switch (mode) {
-    case kF32: return &kF32Tiles[cfg];
-    case kF64: return &kF64Tiles[cfg];
-    case kBF16x3: return &kBF16x3Tiles[cfg];
-    default: return &kBF16x4Tiles[cfg];
-  }
+  const ModeOps* o = ops(mode);
+  if (!o || cfg < 0 || cfg >= o->ncfg) return nullptr;
+  return &o->infos()[cfg];
+}
+
+int gemm_occupancy(int mode, int cfg, int ta, int tb, bool edge) {
+  const ModeOps* o = ops(mode);
+  if (!o || cfg < 0 || cfg >= o->ncfg) return 0;
+  return o->occupancy(cfg, ta, tb, edge);
+}
+
+const GroupFamily* group_families(int mode, int* count) {
+  const ModeOps* o = ops(mode);
+  *count = o ? o->nfam : 0;
+  return o ? o->families() : nullptr;
+}
+
+hipError_t run_grouped(int mode, int fam, const void* table, const int* sched, int blocks, hipStream_t s) {
+  const ModeOps* o = ops(mode);
+  if (!o || fam < 0 || fam >= o->nfam || blocks < 1) return hipErrorInvalidValue;
+  return o->grouped(fam, table, sched, blocks, s);
+}
+
+int grouped_occupancy(int mode, int fam) {
+  const ModeOps* o = ops(mode);
+  if (!o || fam < 0 || fam >= o->nfam) return 1;
+  return o->grouped_occupancy(fam);
+}
+
+hipError_t run_grouped_reduce(int mode, const void* tab, int count, int blocks, hipStream_t s) {
+  if (count < 1 || blocks < 1) return hipSuccess;
+  if (mode == kF64)
+    hipLaunchKernelGGL((grouped_reduce_kernel<double, double>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const GroupReduce<double>*>(tab), count);
+  else
+    hipLaunchKernelGGL((grouped_reduce_kernel<float, float>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const GroupReduce<float>*>(tab), count);
+  return hipGetLastError();
 }
 
 static int grid_for(int64_t total, int threads, int cap) {

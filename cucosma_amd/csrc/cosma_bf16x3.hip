@@ -13,6 +13,11 @@
 // cfg 8: cfg 7 with 16 waves of 64x64 (four waves per SIMD): the pre-split
 //        configuration the heuristic uses (one box, 8192^3: planar kernels 504
 //        and 507 TFLOP/s for cfg 8 and 7; 3072^3: 317 against 301 per call)
+// cfg 9: 256x128, 8 waves of 64x64, ping-pong: cfg 6's thin-N variant
+// cfg 10: 128x256, 8 waves of 64x64, ping-pong: the thin-M variant
+// cfg 11: 128x128, 8 waves of 64x32, ping-pong: the small variant
+// Grouped tile families: {6, 9, 10, 11} in one launch; cfgs 0-5 each on
+// their own.
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -26,67 +31,20 @@ using B3C5 = Tile<kBF16x3, 128, 128, 64, 64, 64>;
 using B3C6 = Tile<kBF16x3, 256, 256, 32, 128, 64, 4>;
 using B3C7 = Planar<kBF16x3, 256, 256, 128, 64>;
 using B3C8 = Planar<kBF16x3, 256, 256, 64, 64>;
+using B3C9 = Tile<kBF16x3, 256, 128, 32, 64, 64, 4>;
+using B3C10 = Tile<kBF16x3, 128, 256, 32, 64, 64, 4>;
+using B3C11 = Tile<kBF16x3, 128, 128, 32, 64, 32, 4>;
 
-extern const TileInfo kBF16x3Tiles[] = {B3C0::info(), B3C1::info(), B3C2::info(), B3C3::info(), B3C4::info(), B3C5::info(), B3C6::info(), B3C7::info(), B3C8::info()};
-extern const int kBF16x3NumTiles = 9;
+using B3Tiles = TileTable<B3C0, B3C1, B3C2, B3C3, B3C4, B3C5, B3C6, B3C7, B3C8, B3C9, B3C10, B3C11>;
+using B3Fams = FamilyTable<Family<kBF16x3, B3Tiles, 6, 9, 10, 11>, Family<kBF16x3, B3Tiles, 0>,
+                           Family<kBF16x3, B3Tiles, 1>, Family<kBF16x3, B3Tiles, 2>, Family<kBF16x3, B3Tiles, 3>,
+                           Family<kBF16x3, B3Tiles, 4>, Family<kBF16x3, B3Tiles, 5>>;
 
-hipError_t run_gemm_bf16x3(const Problem& p) {
-  switch (p.cfg) {
-    case 0: return B3C0::run(p);
-    case 1: return B3C1::run(p);
-    case 2: return B3C2::run(p);
-    case 3: return B3C3::run(p);
-    case 4: return B3C4::run(p);
-    case 5: return B3C5::run(p);
-    case 6: return B3C6::run(p);
-    case 7: return B3C7::run(p);
-    case 8: return B3C8::run(p);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-hipError_t run_grouped_bf16x3(int cfg, const void* table, int count, const int* sched, int blocks,
-                         hipStream_t s) {
-  switch (cfg) {
-    case 0: return B3C0::grouped(table, count, sched, blocks, s);
-    case 1: return B3C1::grouped(table, count, sched, blocks, s);
-    case 2: return B3C2::grouped(table, count, sched, blocks, s);
-    case 3: return B3C3::grouped(table, count, sched, blocks, s);
-    case 4: return B3C4::grouped(table, count, sched, blocks, s);
-    case 5: return B3C5::grouped(table, count, sched, blocks, s);
-    case 6: return B3C6::grouped(table, count, sched, blocks, s);
-    case 7: return hipErrorNotSupported;  // pre-split: no grouped launch
-    case 8: return hipErrorNotSupported;
-    default: return hipErrorInvalidValue;
-  }
-}
-
-int grouped_occupancy_bf16x3(int cfg) {
-  switch (cfg) {
-    case 0: return B3C0::grouped_occupancy();
-    case 1: return B3C1::grouped_occupancy();
-    case 2: return B3C2::grouped_occupancy();
-    case 3: return B3C3::grouped_occupancy();
-    case 4: return B3C4::grouped_occupancy();
-    case 5: return B3C5::grouped_occupancy();
-    case 6: return B3C6::grouped_occupancy();
-    default: return 1;
-  }
-}
-
-int gemm_occupancy_bf16x3(int cfg, int ta, int tb, bool edge) {
-  switch (cfg) {
-    case 0: return B3C0::occupancy(ta, tb, edge);
-    case 1: return B3C1::occupancy(ta, tb, edge);
-    case 2: return B3C2::occupancy(ta, tb, edge);
-    case 3: return B3C3::occupancy(ta, tb, edge);
-    case 4: return B3C4::occupancy(ta, tb, edge);
-    case 5: return B3C5::occupancy(ta, tb, edge);
-    case 6: return B3C6::occupancy(ta, tb, edge);
-    case 7: return B3C7::occupancy(ta, tb, edge);
-    case 8: return B3C8::occupancy(ta, tb, edge);
-    default: return 0;
-  }
+// Host-side dispatch of this mode (a function-local object: a namespace-scope
+// constant would also be emitted for the device).
+const ModeOps* ops_bf16x3() {
+  static const ModeOps ops = make_ops<B3Tiles, B3Fams>();
+  return &ops;
 }
 
 hipError_t run_split_operand(int mode, bool kmajor, int rows, int K, const float* X, int64_t ld, void* planes,

@@ -12,6 +12,10 @@
 // cfg 6: cfg 3 with ping-pong copies (PIPE 4) (deep-K split)
 // cfg 7: cfg 5 with 16 waves of 64x32 (128 VGPRs, four waves per SIMD): the
 //        large-shape tile (139 TFLOP/s at 8192^3, 3-5% above cfg 5 from 4096^3 up)
+// cfg 8: 128x256, 16 waves of 32x64 (PIPE 3): cfg 7's thin-M variant
+// cfg 9: 128x128, 16 waves of 32x32 (PIPE 3): the small variant
+// Grouped tile families: {7, 8, 9} in one launch; every other configuration
+// on its own.
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -24,65 +28,19 @@ using F32C4 = Tile<kF32, 128, 128, 64, 64, 64, 1>;
 using F32C5 = Tile<kF32, 256, 128, 32, 64, 64, 3>;
 using F32C6 = Tile<kF32, 256, 128, 32, 64, 64, 4>;
 using F32C7 = Tile<kF32, 256, 128, 32, 64, 32, 3>;
+using F32C8 = Tile<kF32, 128, 256, 32, 32, 64, 3>;
+using F32C9 = Tile<kF32, 128, 128, 32, 32, 32, 3>;
 
-extern const TileInfo kF32Tiles[] = {F32C0::info(), F32C1::info(), F32C2::info(), F32C3::info(), F32C4::info(), F32C5::info(), F32C6::info(), F32C7::info()};
-extern const int kF32NumTiles = 8;
+using F32Tiles = TileTable<F32C0, F32C1, F32C2, F32C3, F32C4, F32C5, F32C6, F32C7, F32C8, F32C9>;
+using F32Fams = FamilyTable<Family<kF32, F32Tiles, 7, 8, 9>, Family<kF32, F32Tiles, 0>, Family<kF32, F32Tiles, 1>,
+                            Family<kF32, F32Tiles, 2>, Family<kF32, F32Tiles, 3>, Family<kF32, F32Tiles, 4>,
+                            Family<kF32, F32Tiles, 5>, Family<kF32, F32Tiles, 6>>;
 
-hipError_t run_gemm_f32(const Problem& p) {
-  switch (p.cfg) {
-    case 0: return F32C0::run(p);
-    case 1: return F32C1::run(p);
-    case 2: return F32C2::run(p);
-    case 3: return F32C3::run(p);
-    case 4: return F32C4::run(p);
-    case 5: return F32C5::run(p);
-    case 6: return F32C6::run(p);
-    case 7: return F32C7::run(p);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-hipError_t run_grouped_f32(int cfg, const void* table, int count, const int* sched, int blocks,
-                         hipStream_t s) {
-  switch (cfg) {
-    case 0: return F32C0::grouped(table, count, sched, blocks, s);
-    case 1: return F32C1::grouped(table, count, sched, blocks, s);
-    case 2: return F32C2::grouped(table, count, sched, blocks, s);
-    case 3: return F32C3::grouped(table, count, sched, blocks, s);
-    case 4: return F32C4::grouped(table, count, sched, blocks, s);
-    case 5: return F32C5::grouped(table, count, sched, blocks, s);
-    case 6: return F32C6::grouped(table, count, sched, blocks, s);
-    case 7: return F32C7::grouped(table, count, sched, blocks, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-int grouped_occupancy_f32(int cfg) {
-  switch (cfg) {
-    case 0: return F32C0::grouped_occupancy();
-    case 1: return F32C1::grouped_occupancy();
-    case 2: return F32C2::grouped_occupancy();
-    case 3: return F32C3::grouped_occupancy();
-    case 4: return F32C4::grouped_occupancy();
-    case 5: return F32C5::grouped_occupancy();
-    case 6: return F32C6::grouped_occupancy();
-    case 7: return F32C7::grouped_occupancy();
-    default: return 1;
-  }
-}
-
-int gemm_occupancy_f32(int cfg, int ta, int tb, bool edge) {
-  switch (cfg) {
-    case 0: return F32C0::occupancy(ta, tb, edge);
-    case 1: return F32C1::occupancy(ta, tb, edge);
-    case 2: return F32C2::occupancy(ta, tb, edge);
-    case 3: return F32C3::occupancy(ta, tb, edge);
-    case 4: return F32C4::occupancy(ta, tb, edge);
-    case 5: return F32C5::occupancy(ta, tb, edge);
-    case 6: return F32C6::occupancy(ta, tb, edge);
-    case 7: return F32C7::occupancy(ta, tb, edge);
-    default: return 0;
-  }
+// Host-side dispatch of this mode (a function-local object: a namespace-scope
+// constant would also be emitted for the device).
+const ModeOps* ops_f32() {
+  static const ModeOps ops = make_ops<F32Tiles, F32Fams>();
+  return &ops;
 }
 
 }  // namespace cosma

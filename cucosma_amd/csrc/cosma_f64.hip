@@ -9,6 +9,11 @@
 // cfg 5: 128x128 with a 32-deep K chunk, 8 waves of 64x32
 // cfg 6: 256x128, 8 waves of 64x64
 // cfg 7: cfg 3 with ping-pong copies (PIPE 4): the large-shape tile
+// cfg 8: 128x64, 8 waves of 32x32 (PIPE 4, 4 waves per SIMD): cfg 7's thin-N variant
+// cfg 9: 64x128, 8 waves of 32x32 (PIPE 4): the thin-M variant
+// cfg 10: 64x64, 8 waves of 32x16 (PIPE 4): the small variant
+// Grouped tile families: {7, 8, 9, 10} in one launch; every other
+// configuration on its own.
 #include "cosma_launch.hpp"
 
 namespace cosma {
@@ -21,65 +26,20 @@ using F64C4 = Tile<kF64, 64, 64, 32, 32, 32>;
 using F64C5 = Tile<kF64, 128, 128, 32, 64, 32>;
 using F64C6 = Tile<kF64, 256, 128, 16, 64, 64>;
 using F64C7 = Tile<kF64, 128, 128, 16, 64, 32, 4, 4>;
+using F64C8 = Tile<kF64, 128, 64, 16, 32, 32, 4, 4>;
+using F64C9 = Tile<kF64, 64, 128, 16, 32, 32, 4, 4>;
+using F64C10 = Tile<kF64, 64, 64, 16, 32, 16, 4, 4>;
 
-extern const TileInfo kF64Tiles[] = {F64C0::info(), F64C1::info(), F64C2::info(), F64C3::info(), F64C4::info(), F64C5::info(), F64C6::info(), F64C7::info()};
-extern const int kF64NumTiles = 8;
+using F64Tiles = TileTable<F64C0, F64C1, F64C2, F64C3, F64C4, F64C5, F64C6, F64C7, F64C8, F64C9, F64C10>;
+using F64Fams = FamilyTable<Family<kF64, F64Tiles, 7, 8, 9, 10>, Family<kF64, F64Tiles, 0>, Family<kF64, F64Tiles, 1>,
+                            Family<kF64, F64Tiles, 2>, Family<kF64, F64Tiles, 3>, Family<kF64, F64Tiles, 4>,
+                            Family<kF64, F64Tiles, 5>, Family<kF64, F64Tiles, 6>>;
 
-hipError_t run_gemm_f64(const Problem& p) {
-  switch (p.cfg) {
-    case 0: return F64C0::run(p);
-    case 1: return F64C1::run(p);
-    case 2: return F64C2::run(p);
-    case 3: return F64C3::run(p);
-    case 4: return F64C4::run(p);
-    case 5: return F64C5::run(p);
-    case 6: return F64C6::run(p);
-    case 7: return F64C7::run(p);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-hipError_t run_grouped_f64(int cfg, const void* table, int count, const int* sched, int blocks,
-                         hipStream_t s) {
-  switch (cfg) {
-    case 0: return F64C0::grouped(table, count, sched, blocks, s);
-    case 1: return F64C1::grouped(table, count, sched, blocks, s);
-    case 2: return F64C2::grouped(table, count, sched, blocks, s);
-    case 3: return F64C3::grouped(table, count, sched, blocks, s);
-    case 4: return F64C4::grouped(table, count, sched, blocks, s);
-    case 5: return F64C5::grouped(table, count, sched, blocks, s);
-    case 6: return F64C6::grouped(table, count, sched, blocks, s);
-    case 7: return F64C7::grouped(table, count, sched, blocks, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-int grouped_occupancy_f64(int cfg) {
-  switch (cfg) {
-    case 0: return F64C0::grouped_occupancy();
-    case 1: return F64C1::grouped_occupancy();
-    case 2: return F64C2::grouped_occupancy();
-    case 3: return F64C3::grouped_occupancy();
-    case 4: return F64C4::grouped_occupancy();
-    case 5: return F64C5::grouped_occupancy();
-    case 6: return F64C6::grouped_occupancy();
-    case 7: return F64C7::grouped_occupancy();
-    default: return 1;
-  }
-}
-
-int gemm_occupancy_f64(int cfg, int ta, int tb, bool edge) {
-  switch (cfg) {
-    case 0: return F64C0::occupancy(ta, tb, edge);
-    case 1: return F64C1::occupancy(ta, tb, edge);
-    case 2: return F64C2::occupancy(ta, tb, edge);
-    case 3: return F64C3::occupancy(ta, tb, edge);
-    case 4: return F64C4::occupancy(ta, tb, edge);
-    case 5: return F64C5::occupancy(ta, tb, edge);
-    case 6: return F64C6::occupancy(ta, tb, edge);
-    case 7: return F64C7::occupancy(ta, tb, edge);
-    default: return 0;
-  }
+// Host-side dispatch of this mode (a function-local object: a namespace-scope
+// constant would also be emitted for the device).
+const ModeOps* ops_f64() {
+  static const ModeOps ops = make_ops<F64Tiles, F64Fams>();
+  return &ops;
 }
 
 }  // namespace cosma

@@ -26,6 +26,7 @@ struct Problem {
   int split;
   int cfg;
   int group_m;
+  int xcds;  // XCDs of the device (workgroups are dealt to them round-robin by id)
   int act;  // output activation: 0 none, 1 ReLU, 2 sigmoid
   void* ws;
   void* planes;  // pre-split tile configurations: room for the planes not given below
@@ -47,19 +48,44 @@ struct TileInfo {
 };
 
 // One problem of a grouped launch, as the grouped kernel reads it from
-// device memory.  tile_start: index of the problem's first output tile in
-// the launch's tile numbering (ascending over the table).
+// device memory.  variant: index of the problem's tile in the launch's tile
+// family.  split > 1: the tiles of K slice z write raw partials to
+// ws + z*M*N (accumulator type, row length N), reduced after the launch.  K
+// is 0 for alpha == 0 (the tiles then only scale C by beta).  vec_a / vec_b:
+// the operand's rows start on 16-byte boundaries.
 template <typename G>
 struct GroupEntry {
   const G* A;
   const G* B;
   G* C;
+  void* ws;
   int64_t lda, ldb, ldc;
   int M, N, K;
-  int tiles_n;
-  int tile_start;
-  int pad_;
+  int variant;
+  int tiles_m, tiles_n;
+  int split, kslice;
+  int vec_a, vec_b;
   double alpha, beta;
+};
+
+// One split member of a grouped launch for the fixed-order reduction:
+// C = alpha * sum_z ws[z] (+ beta * C).  block0: first reduction block of the
+// member (256 elements per block, ascending over the table).
+template <typename G>
+struct GroupReduce {
+  const void* ws;
+  G* C;
+  int64_t ldc;
+  int M, N, S;
+  int block0;
+  double alpha, beta;
+};
+
+// A grouped tile family: tile configurations (indices of the mode's gemm
+// tile table) that one persistent kernel runs side by side, one per problem.
+struct GroupFamily {
+  int ncfg;
+  int cfgs[4];
 };
 
 // Bytes of bf16 planes a pre-split tile configuration needs: (M + N) rows of
@@ -76,20 +102,23 @@ constexpr int kPipePlanar = 5;
 int num_tile_cfgs(int mode);
 const TileInfo* tile_info(int mode, int cfg);  // nullptr when out of range
 
-// Launch the main kernel (and, when split > 1, the fixed-order reduction).
-hipError_t run_gemm_f32(const Problem& p);
-hipError_t run_gemm_f64(const Problem& p);
-hipError_t run_gemm_bf16x3(const Problem& p);
-hipError_t run_gemm_bf16x4(const Problem& p);
+// Launch the main kernel of tile configuration p.cfg of p.mode (and, when
+// split > 1, the fixed-order reduction).
+hipError_t run_gemm(const Problem& p);
 
-// Grouped persistent launch of `blocks` workgroups over `count` table
-// entries (device memory) of tile config cfg; sched (device memory) holds
-// blocks + 1 offsets and then, for each workgroup's tiles in order, the pair
-// (table index of the problem, tile index within the problem).
-hipError_t run_grouped(int mode, int cfg, const void* table, int count, const int* sched, int blocks,
-                       hipStream_t s);
-// Workgroups of the grouped kernel of (mode, cfg) that fit on one CU.
-int grouped_occupancy(int mode, int cfg);
+// Tile families of a mode's grouped launches (every non-pre-split tile
+// configuration belongs to exactly one); *count receives their number.
+const GroupFamily* group_families(int mode, int* count);
+// Grouped persistent launch of family fam: `blocks` workgroups over the
+// table (device memory); sched (device memory) holds blocks + 1 offsets and
+// then, for each workgroup's tiles in order, the pair (table index of the
+// problem, tile index within the problem).
+hipError_t run_grouped(int mode, int fam, const void* table, const int* sched, int blocks, hipStream_t s);
+// Workgroups of family fam's grouped kernel that fit on one CU.
+int grouped_occupancy(int mode, int fam);
+// Fixed-order reduction of the split members of a grouped launch: `count`
+// GroupReduce entries (device memory), `blocks` blocks of 256 threads.
+hipError_t run_grouped_reduce(int mode, const void* tab, int count, int blocks, hipStream_t s);
 // Workgroups of one gemm_kernel instantiation that fit on one CU (0 if the
 // query fails or the layout is not built).
 int gemm_occupancy(int mode, int cfg, int ta, int tb, bool edge);

@@ -129,7 +129,8 @@ struct GemmParams {
   int tiles_m, tiles_n;
   int split;   // number of K slices
   int kslice;  // K per slice, a multiple of the tile's BK
-  int group_m; // tile rows per group in the grouped tile order
+  int group_m; // tile rows per group in the grouped tile order (1..tiles_m)
+  int xcds;    // XCDs of the device: workgroups are dealt to them round-robin by id
   int vec_a;   // 1: A's rows start on 16-byte boundaries (vector loads allowed)
   int vec_b;   // 1: same for B
   int act;     // element-wise activation applied to the final C values (kActNone/ReLU/Sigmoid)
@@ -206,17 +207,17 @@ __device__ __forceinline__ void split_bf16x4(f32x4 v, bf16x4& hi, bf16x4& lo) {
 }
 
 // Linear block id -> (tile row, tile col, split index).
-// Workgroups are handed to the 8 XCDs round-robin by id, so ids b, b+8, ...
-// share an L2.  The first remap gives each XCD a contiguous range of logical
-// ids; the grouped order then walks group_m tile rows column by column, so the
-// workgroups resident on one XCD at a time share A row panels and B column
-// panels.  Split index is the slowest-varying coordinate.
+// Workgroups are handed to the device's xcds XCDs round-robin by id, so ids
+// b, b + xcds, ... share an L2.  The first remap gives each XCD a contiguous
+// range of logical ids; the grouped order then walks group_m tile rows
+// column by column, so the workgroups resident on one XCD at a time share A
+// row panels and B column panels.  Split index is the slowest-varying
+// coordinate.  group_m is at most tiles_m (the host clamps it).
 __device__ __forceinline__ void map_block(int bid, int nblk, int tiles_m, int tiles_n,
-                                          int group_m, int& tm, int& tn, int& z) {
-  constexpr int kXcd = 8;
-  const int full = (nblk / kXcd) * kXcd;
+                                          int group_m, int xcds, int& tm, int& tn, int& z) {
+  const int full = xcds > 1 ? (nblk / xcds) * xcds : 0;
   int lid = bid;
-  if (bid < full) lid = (bid % kXcd) * (full / kXcd) + bid / kXcd;
+  if (bid < full) lid = (bid % xcds) * (full / xcds) + bid / xcds;
   const int tiles = tiles_m * tiles_n;
   z = lid / tiles;
   const int t = lid - z * tiles;
@@ -777,60 +778,149 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
   int tm, tn, z;
-  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
+  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, p.xcds, tm, tn, z);
   gemm_tile<MODE, BM, BN, BK, WM, WN, PIPE, TA, TB, EDGE>(p, tm, tn, z, lds);
 }
 
 // ---------------------------------------------------------------------------
 // Grouped GEMM: many independent row-major problems in one persistent launch.
-// The table lists every problem.  sched holds the host-computed schedule:
-// sched[w] .. sched[w + 1] index the tiles of workgroup w, each a pair
-// (problem, tile within the problem) at sched[gridDim.x + 1 + 2i] and the
-// next int (a longest-first assignment that balances the K-steps per
-// workgroup; the pair costs one table load per tile, where a search of the
-// table for a global tile index took a chain of dependent loads).  Each tile
-// is computed with the same gemm_tile as the single-problem kernel (split 1,
-// bounds-checked edges), so the schedule never changes a result.
+// A launch runs one tile family: tile variants (Vs) with one workgroup size,
+// each problem of the table using one of them (GroupEntry::variant), so thin
+// outputs take thin tiles inside the same launch and under the same
+// schedule.  sched holds the host-computed schedule: sched[w] .. sched[w + 1]
+// index the tiles of workgroup w, each a pair (problem, tile within the
+// problem) at sched[gridDim.x + 1 + 2i] and the next int (a longest-first
+// assignment that balances the work per workgroup).  A problem's tiles are
+// numbered z * tiles_m * tiles_n + tm * tiles_n + tn over its split-K slices
+// z.  Each tile is computed with the same gemm_tile as the single-problem
+// kernel, so a problem's result is bitwise the single launch's of the same
+// tile configuration and split: slices of a split member write their raw
+// partials to the member's workspace (ws[z][M][N]), which
+// grouped_reduce_kernel sums in ascending z after the launch.  A tile that
+// lies inside its problem (and whose operands allow 16-byte loads and whose
+// K is a whole number of chunks) runs the unchecked loaders; the others the
+// bounds-checked ones -- the same values either way.
 // ---------------------------------------------------------------------------
-template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, int OCC>
-__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amdgpu_waves_per_eu(OCC)))
-    grouped_kernel(const GroupEntry<typename Traits<MODE>::G>* __restrict__ table, int count,
-                   const int* __restrict__ sched) {
-  using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int PIPE_>
+struct TileV {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_, PIPE = PIPE_;
+  static constexpr int NT = 64 * (BM / WM) * (BN / WN);
+};
+
+template <int MODE, class V>
+__device__ __forceinline__ void grouped_tile(const GroupEntry<typename Traits<MODE>::G>& e, int local,
+                                             typename Traits<MODE>::S* lds) {
   using E = typename Traits<MODE>::E;
-  (void)count;  // table entries; the host writes only indices below it into sched
-  __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[2 * Sh::STAGE];
+  GemmParams<MODE> p;
+  p.A = e.A;
+  p.B = e.B;
+  p.C = e.C;
+  p.ws = static_cast<E*>(e.ws);
+  p.act = kActNone;
+  p.lda = e.lda;
+  p.ldb = e.ldb;
+  p.ldc = e.ldc;
+  p.M = e.M;
+  p.N = e.N;
+  p.K = e.K;
+  p.tiles_m = e.tiles_m;
+  p.tiles_n = e.tiles_n;
+  p.split = e.split;
+  p.kslice = e.kslice;
+  p.group_m = 1;
+  p.xcds = 1;
+  p.vec_a = e.vec_a;
+  p.vec_b = e.vec_b;
+  p.alpha = (E)e.alpha;
+  p.beta = (E)e.beta;
+  const int per = e.tiles_m * e.tiles_n;
+  const int z = local / per;
+  const int t = local - z * per;
+  const int tm = t / e.tiles_n;
+  const int tn = t - tm * e.tiles_n;
+  const bool inside = e.vec_a && e.vec_b && (tm + 1) * V::BM <= e.M && (tn + 1) * V::BN <= e.N &&
+                      e.K % V::BK == 0;
+  if (inside)
+    gemm_tile<MODE, V::BM, V::BN, V::BK, V::WM, V::WN, V::PIPE, false, false, false>(p, tm, tn, z, lds);
+  else
+    gemm_tile<MODE, V::BM, V::BN, V::BK, V::WM, V::WN, V::PIPE, false, false, true>(p, tm, tn, z, lds);
+}
+
+template <int MODE, class V0, class... Vs>
+__device__ __forceinline__ void grouped_dispatch(int v, const GroupEntry<typename Traits<MODE>::G>& e,
+                                                 int local, typename Traits<MODE>::S* lds) {
+  if constexpr (sizeof...(Vs) == 0) {
+    (void)v;
+    grouped_tile<MODE, V0>(e, local, lds);
+  } else {
+    if (v == 0)
+      grouped_tile<MODE, V0>(e, local, lds);
+    else
+      grouped_dispatch<MODE, Vs...>(v - 1, e, local, lds);
+  }
+}
+
+template <class V0, class... Vs>
+struct FamilyNT {
+  static constexpr int value = V0::NT;
+  static_assert(((Vs::NT == V0::NT) && ...), "every variant of a family has one workgroup size");
+};
+
+// LDS elements of two stages of the largest variant.
+template <int MODE, class... Vs>
+constexpr int family_lds_elems() {
+  int m = 0;
+  ((m = m > 2 * TileShape<MODE, Vs::BM, Vs::BN, Vs::BK, Vs::WM, Vs::WN>::STAGE
+            ? m
+            : 2 * TileShape<MODE, Vs::BM, Vs::BN, Vs::BK, Vs::WM, Vs::WN>::STAGE),
+   ...);
+  return m;
+}
+
+template <int MODE, int OCC, class... Vs>
+__global__ void __launch_bounds__(FamilyNT<Vs...>::value) __attribute__((amdgpu_waves_per_eu(OCC)))
+    grouped_kernel(const GroupEntry<typename Traits<MODE>::G>* __restrict__ table,
+                   const int* __restrict__ sched) {
+  __shared__ __attribute__((aligned(16))) typename Traits<MODE>::S lds[family_lds_elems<MODE, Vs...>()];
   const int* order = sched + gridDim.x + 1;
   const int end = sched[blockIdx.x + 1];
   for (int i = sched[blockIdx.x]; i < end; ++i) {
     const int prob = order[2 * (int64_t)i];
     const int local = order[2 * (int64_t)i + 1];
     const GroupEntry<typename Traits<MODE>::G>& e = table[prob];
-    GemmParams<MODE> p;
-    p.A = e.A;
-    p.B = e.B;
-    p.C = e.C;
-    p.ws = nullptr;
-    p.act = kActNone;
-    p.lda = e.lda;
-    p.ldb = e.ldb;
-    p.ldc = e.ldc;
-    p.M = e.M;
-    p.N = e.N;
-    p.K = e.K;
-    p.tiles_m = (e.M + BM - 1) / BM;
-    p.tiles_n = e.tiles_n;
-    p.split = 1;
-    p.kslice = ((e.K + BK - 1) / BK) * BK;
-    p.group_m = 1;
-    p.vec_a = ((uintptr_t)e.A % 16 == 0) && (e.lda % Traits<MODE>::GVEC == 0);
-    p.vec_b = ((uintptr_t)e.B % 16 == 0) && (e.ldb % Traits<MODE>::GVEC == 0);
-    p.alpha = (E)e.alpha;
-    p.beta = (E)e.beta;
-    const int tm = local / e.tiles_n;
-    const int tn = local - tm * e.tiles_n;
-    gemm_tile<MODE, BM, BN, BK, WM, WN, PIPE, false, false, true>(p, tm, tn, 0, lds);
+    grouped_dispatch<MODE, Vs...>(e.variant, e, local, lds);
   }
+}
+
+// Fixed-order reduction of the split members of a grouped launch: for each
+// member, C = alpha * sum_{z ascending} ws[z] (+ beta * C), the arithmetic of
+// splitk_reduce_kernel (no activation).  Block b serves member i with
+// tab[i].block0 <= b < tab[i + 1].block0 (256 elements per block).
+template <typename G, typename E>
+__global__ void __launch_bounds__(256)
+    grouped_reduce_kernel(const GroupReduce<G>* __restrict__ tab, int count) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = count - 1;  // last member whose block0 <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].block0 <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const GroupReduce<G>& r = tab[lo];
+  const int64_t total = (int64_t)r.M * r.N;
+  const int64_t e = (int64_t)(b - r.block0) * 256 + threadIdx.x;
+  if (e >= total) return;
+  const E* p = static_cast<const E*>(r.ws) + e;
+  E v = E(0);
+  for (int z = 0; z < r.S; ++z) v += p[(int64_t)z * total];
+  const int row = (int)(e / r.N);
+  const int col = (int)(e - (int64_t)row * r.N);
+  G* cp = r.C + (int64_t)row * r.ldc + col;
+  E out = (E)r.alpha * v;
+  if ((E)r.beta != E(0)) out = fma_e((E)r.beta, *cp, out);
+  *cp = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -952,7 +1042,7 @@ __device__ __forceinline__ void planar_tile(const PlanarParams<MODE>& pp, __bf16
   static_assert(L::NC == 8, "128-byte LDS rows");
   const GemmParams<MODE>& p = pp.g;
   int tm, tn, z;
-  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, tm, tn, z);
+  map_block(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, p.group_m, p.xcds, tm, tn, z);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;

@@ -1,6 +1,9 @@
 // cosma_launch.hpp -- instantiation and launch of gemm_kernel for one tile
 // configuration.  Included by the per-mode translation units only.
 #pragma once
+#include <algorithm>
+#include <tuple>
+
 #include "cosma_internal.hpp"
 #include "cosma_kernels.hpp"
 
@@ -9,6 +12,9 @@ namespace cosma {
 template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE = 0, int OCC = 1>
 struct Tile {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
+  using V = TileV<BM, BN, BK, WM, WN, PIPE>;  // as a variant of a grouped tile family
+  static constexpr int kOcc = OCC;
+  static constexpr bool kGrouped = true;
   static constexpr TileInfo info() {
     return TileInfo{BM, BN, BK, WM, WN, Sh::NT,
                     (int)(2 * Sh::STAGE * sizeof(typename Traits<MODE>::S)), PIPE};
@@ -19,22 +25,6 @@ struct Tile {
     hipLaunchKernelGGL((gemm_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC, TA, TB, EDGE>), dim3(nblk),
                        dim3(Sh::NT), 0, s, gp);
     return hipGetLastError();
-  }
-
-  static hipError_t grouped(const void* table, int count, const int* sched, int blocks, hipStream_t s) {
-    using G = typename Traits<MODE>::G;
-    hipLaunchKernelGGL((grouped_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC>), dim3(blocks), dim3(Sh::NT), 0, s,
-                       static_cast<const GroupEntry<G>*>(table), count, sched);
-    return hipGetLastError();
-  }
-
-  static int grouped_occupancy() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&grouped_kernel<MODE, BM, BN, BK, WM, WN, PIPE, OCC>), Sh::NT,
-            0) != hipSuccess)
-      return 1;
-    return n > 0 ? n : 1;
   }
 
   // Workgroups of one gemm_kernel instantiation that fit on a CU.
@@ -73,7 +63,8 @@ struct Tile {
     gp.split = pr.split;
     const int per = (pr.K + pr.split - 1) / pr.split;
     gp.kslice = ((per + BK - 1) / BK) * BK;
-    gp.group_m = pr.group_m > 0 ? pr.group_m : 8;
+    gp.group_m = std::min(pr.group_m > 0 ? pr.group_m : 8, gp.tiles_m);
+    gp.xcds = pr.xcds > 0 ? pr.xcds : 1;
     gp.act = pr.act;
     gp.alpha = (E)pr.alpha;
     gp.beta = (E)pr.beta;
@@ -108,6 +99,7 @@ struct Tile {
 // planar_kernel (and the reduction when split > 1).
 template <int MODE, int BM, int BN, int WM, int WN>
 struct Planar {
+  static constexpr bool kGrouped = false;  // no grouped launch over pre-split operands
   static constexpr int kThreads = 64 * (BM / WM) * (BN / WN);
   static constexpr TileInfo info() {
     return TileInfo{BM, BN, 32, WM, WN, kThreads, (int)(2 * (BM + BN) * 64 * sizeof(__bf16)), kPipePlanar};
@@ -176,7 +168,8 @@ struct Planar {
     gp.split = pr.split;
     const int per = (pr.K + pr.split - 1) / pr.split;
     gp.kslice = ((per + 31) / 32) * 32;
-    gp.group_m = pr.group_m > 0 ? pr.group_m : 8;
+    gp.group_m = std::min(pr.group_m > 0 ? pr.group_m : 8, gp.tiles_m);
+    gp.xcds = pr.xcds > 0 ? pr.xcds : 1;
     gp.vec_a = 1;
     gp.vec_b = 1;
     gp.act = pr.act;
@@ -202,5 +195,94 @@ struct Planar {
                           pr.split, (float)pr.alpha, (float)pr.beta, pr.act, pr.stream);
   }
 };
+
+// A mode's tile table: configuration cfg is the cfg-th type.  Dispatch by
+// index folds over the types.
+template <class... Ts>
+struct TileTable {
+  static constexpr int kCount = sizeof...(Ts);
+  template <int I>
+  using at = std::tuple_element_t<I, std::tuple<Ts...>>;
+  static const TileInfo* infos() {
+    static const TileInfo t[] = {Ts::info()...};
+    return t;
+  }
+  static hipError_t run(int cfg, const Problem& p) {
+    hipError_t e = hipErrorInvalidValue;
+    int i = 0;
+    (void)((i++ == cfg && ((e = Ts::run(p)), true)) || ...);
+    return e;
+  }
+  static int occupancy(int cfg, int ta, int tb, bool edge) {
+    int r = 0, i = 0;
+    (void)((i++ == cfg && ((r = Ts::occupancy(ta, tb, edge)), true)) || ...);
+    return r;
+  }
+};
+
+// Grouped tile family: the configurations Is of Table that one persistent
+// grouped_kernel runs side by side (one per problem).  They share the
+// workgroup size; the family takes the first one's register cap (OCC).
+template <int MODE, class Table, int... Is>
+struct Family {
+  static_assert(sizeof...(Is) >= 1 && sizeof...(Is) <= 4, "one to four variants");
+  using First = typename Table::template at<std::get<0>(std::make_tuple(Is...))>;
+  static constexpr int kOcc = First::kOcc;
+  static constexpr int kThreads = FamilyNT<typename Table::template at<Is>::V...>::value;
+  static_assert((Table::template at<Is>::kGrouped && ...), "no pre-split configuration in a family");
+  static constexpr GroupFamily info() { return GroupFamily{(int)sizeof...(Is), {Is...}}; }
+  static auto kernel() { return &grouped_kernel<MODE, kOcc, typename Table::template at<Is>::V...>; }
+  static hipError_t launch(const void* table, const int* sched, int blocks, hipStream_t s) {
+    using G = typename Traits<MODE>::G;
+    hipLaunchKernelGGL((grouped_kernel<MODE, kOcc, typename Table::template at<Is>::V...>), dim3(blocks),
+                       dim3(kThreads), 0, s, static_cast<const GroupEntry<G>*>(table), sched);
+    return hipGetLastError();
+  }
+  static int occupancy() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(kernel()), kThreads, 0) !=
+        hipSuccess)
+      return 1;
+    return n > 0 ? n : 1;
+  }
+};
+
+template <class... Fs>
+struct FamilyTable {
+  static constexpr int kCount = sizeof...(Fs);
+  static const GroupFamily* infos() {
+    static const GroupFamily t[] = {Fs::info()...};
+    return t;
+  }
+  static hipError_t launch(int fam, const void* table, const int* sched, int blocks, hipStream_t s) {
+    hipError_t e = hipErrorInvalidValue;
+    int i = 0;
+    (void)((i++ == fam && ((e = Fs::launch(table, sched, blocks, s)), true)) || ...);
+    return e;
+  }
+  static int occupancy(int fam) {
+    int r = 1, i = 0;
+    (void)((i++ == fam && ((r = Fs::occupancy()), true)) || ...);
+    return r;
+  }
+};
+
+// The entry points one mode's translation unit exports to cosma_aux.hip.
+struct ModeOps {
+  int ncfg;
+  const TileInfo* (*infos)();
+  hipError_t (*run)(int, const Problem&);
+  int (*occupancy)(int, int, int, bool);
+  int nfam;
+  const GroupFamily* (*families)();
+  hipError_t (*grouped)(int, const void*, const int*, int, hipStream_t);
+  int (*grouped_occupancy)(int);
+};
+
+template <class Tiles, class Fams>
+ModeOps make_ops() {
+  return ModeOps{Tiles::kCount, &Tiles::infos, &Tiles::run, &Tiles::occupancy,
+                 Fams::kCount, &Fams::infos, &Fams::launch, &Fams::occupancy};
+}
 
 }  // namespace cosma

@@ -23,6 +23,17 @@ def _stream_ptr(stream) -> int:
     return int(stream.cuda_stream)
 
 
+def _check_operand(t: torch.Tensor, name: str, dt, mode: str, device) -> None:
+    """dtype of the mode, on the GPU (on `device` when given): the kernels
+    read and write raw device pointers."""
+    if t.dtype != dt:
+        raise TypeError(f"{name} must be {dt} for mode {mode}, got {t.dtype}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, A on {device}")
+
+
 def _ld(t: torch.Tensor, name: str) -> int:
     if t.dim() != 2:
         raise ValueError(f"{name} must be 2-D, got {tuple(t.shape)}")
@@ -68,11 +79,8 @@ def gemm(
     if epilogue not in EPILOGUES:
         raise ValueError(f"unknown epilogue {epilogue!r}")
     dt = _DTYPE[mode]
-    for t, name in ((A, "A"), (B, "B")):
-        if t.dtype != dt:
-            raise TypeError(f"{name} must be {dt} for mode {mode}, got {t.dtype}")
-        if not t.is_cuda:
-            raise ValueError(f"{name} must be a device tensor")
+    _check_operand(A, "A", dt, mode, None)
+    _check_operand(B, "B", dt, mode, A.device)
     m, k = (A.shape[1], A.shape[0]) if trans_a else (A.shape[0], A.shape[1])
     kb, n = (B.shape[1], B.shape[0]) if trans_b else (B.shape[0], B.shape[1])
     if k != kb:
@@ -83,6 +91,7 @@ def gemm(
         C_out = torch.empty((m, n), dtype=dt, device=A.device)
     if C_out.dtype != dt or tuple(C_out.shape) != (m, n):
         raise ValueError(f"C must be {dt} of shape {(m, n)}")
+    _check_operand(C_out, "C", dt, mode, A.device)
     d = GemmDesc()
     lib.cosma_gemm_desc_init(C.byref(d))
     d.mode = MODES[mode]
@@ -94,9 +103,15 @@ def gemm(
     d.alpha, d.beta = float(alpha), float(beta)
     d.split_k, d.tile_cfg, d.group_m = int(split_k), int(tile_cfg), int(group_m)
     d.epilogue = EPILOGUES[epilogue]
-    for t, name in ((a_planes, "a_planes"), (b_planes, "b_planes")):
-        if t is not None and not (t.is_cuda and t.is_contiguous()):
-            raise ValueError(f"{name} must be a contiguous device tensor from split_operand")
+    for t, name, rows in ((a_planes, "a_planes", m), (b_planes, "b_planes", n)):
+        if t is None:
+            continue
+        if not (t.is_cuda and t.is_contiguous()) or t.device != A.device:
+            raise ValueError(f"{name} must be a contiguous tensor on {A.device} from split_operand")
+        need = int(lib.cosma_planes_bytes(rows, k))
+        if t.numel() * t.element_size() < need:
+            raise ValueError(f"{name} holds {t.numel() * t.element_size()} bytes, the planes of a "
+                             f"{rows} x {k} operand need {need}")
     d.a_planes = a_planes.data_ptr() if a_planes is not None else None
     d.b_planes = b_planes.data_ptr() if b_planes is not None else None
     if workspace is not None:

@@ -21,9 +21,8 @@ Configurations:
            entries x1e6, C normal(0,1), alpha = beta = 1
   grouped  512 problems, M/N/K log-uniform in [32, 2048] rounded to 32, seed 3,
            fp32, bf16x3, bf16x4 and fp64 (FP64 storage for fp64):
-           one hipGraph replay of the library's default grouped plan vs one
-           launch per problem, and every assignment of 64x64 / 128x128 /
-           large tiles to three output-size classes for comparison
+           one hipGraph replay of the library's default grouped plan, of the
+           grouped autotuner's plan, and one launch per problem
   autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
@@ -214,7 +213,9 @@ def grouped_sizes(count=512, seed=3, lo=32, hi=2048):
 def cfg_grouped(cc, torch, args, emit):
     sizes = grouped_sizes()
     flops = sum(2.0 * m * n * k for m, n, k in sizes)
-    for mode in ("fp32", "bf16x3", "bf16x4", "fp64"):
+    modes = args.grouped_modes.split(",") if getattr(args, "grouped_modes", None) else \
+        ("fp32", "bf16x3", "bf16x4", "fp64")
+    for mode in modes:
         dt = torch.float64 if mode == "fp64" else torch.float32
         g = torch.Generator(device="cuda").manual_seed(3)
         As = [torch.rand((m, k), device="cuda", generator=g, dtype=dt) * 2 - 1 for m, n, k in sizes]
@@ -235,26 +236,34 @@ def cfg_grouped(cc, torch, args, emit):
             torch.cuda.synchronize()
             return ev[0].elapsed_time(ev[1]) / reps
 
-        # The library's default plan (one tile configuration for the batch).
+        def describe(plan):
+            tiles = cc.tile_configs(cc._lib.MODES[mode])
+            use = {}
+            for c, s in plan.choices():
+                key = f"{tiles[c]['BM']}x{tiles[c]['BN']}" + (f"/sk{s}" if s > 1 else "")
+                use[key] = use.get(key, 0) + 1
+            return use
+
+        # The library's default plan: each problem on the first tile family's
+        # variant with the least weighted padding, one persistent launch.
         plan = cc.GroupedGemm(As, Bs, Cs, mode=mode)
         ms_graph = time_plan(plan, args.runs)
-        info = plan.info()
+        info, use = plan.info(), describe(plan)
+        padded = sum(2.0 * -(-m // cc.tile_configs(cc._lib.MODES[mode])[c]["BM"])
+                     * cc.tile_configs(cc._lib.MODES[mode])[c]["BM"]
+                     * -(-n // cc.tile_configs(cc._lib.MODES[mode])[c]["BN"])
+                     * cc.tile_configs(cc._lib.MODES[mode])[c]["BN"] * k
+                     for (m, n, k), (c, s) in zip(sizes, plan.choices()))
         plan.close()
-        # Against it: every per-size-class assignment of the candidate tiles
-        # (largest output side below 256, below 1024, from 1024).
-        edges = (256, 1024)
-        bucket = [0 if max(m, n) < edges[0] else (1 if max(m, n) < edges[1] else 2)
-                  for m, n, k in sizes]
-        large = {"fp32": 7, "bf16x3": 6, "bf16x4": 3, "fp64": 7}[mode]
-        cand = (1, 0, large)
-        tried = []
-        for c0 in () if args.no_search else cand:
-            for c1 in cand:
-                for c2 in cand:
-                    p = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=[(c0, c1, c2)[b] for b in bucket])
-                    tried.append(((c0, c1, c2), time_plan(p, 10)))
-                    p.close()
-        best = min(tried, key=lambda x: x[1]) if tried else ((), float("nan"))
+        # The grouped autotuner's per-size-bucket choices.
+        tuned = None
+        if not args.no_search:
+            res = cc.grouped_autotune(As, Bs, Cs, mode=mode, warmup=3, reps=10)
+            tp = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=res["tile_cfgs"], splits=res["splits"])
+            ms_tuned = time_plan(tp, args.runs)
+            tuned = dict(ms=ms_tuned, tflops=flops / ms_tuned / 1e9, tiles=describe(tp),
+                         candidates=len(res["tried"]))
+            tp.close()
         # one launch per problem (library heuristic per shape)
         for A, B, Cm in zip(As, Bs, Cs):
             cc.gemm(A, B, Cm, mode=mode)
@@ -268,10 +277,8 @@ def cfg_grouped(cc, torch, args, emit):
         torch.cuda.synchronize()
         ms_loop = ev[0].elapsed_time(ev[1]) / reps
         emit(dict(config="grouped", mode=mode, problems=len(sizes), gflop=flops / 1e9,
-                  buckets=info["buckets"], bucket_edges=list(edges),
-                  searched={",".join(map(str, c)): round(t, 4) for c, t in tried},
-                  best_searched=",".join(map(str, best[0])), ms_best_searched=best[1],
-                  ms_graph=ms_graph, tflops_graph=flops / ms_graph / 1e9,
+                  padded_mfma_gflop=padded / 1e9, buckets=info["buckets"], tiles_used=use,
+                  ms_graph=ms_graph, tflops_graph=flops / ms_graph / 1e9, tuned=tuned,
                   ms_per_problem_launches=ms_loop, tflops_per_problem_launches=flops / ms_loop / 1e9,
                   speedup=ms_loop / ms_graph,
                   pct_peak=100 * flops / ms_graph / 1e9 / PEAK[mode]))
@@ -324,7 +331,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tune-db", help="apply this autotuner DB before running")
     ap.add_argument("--out", help="append JSON lines here")
     ap.add_argument("--no-search", action="store_true",
-                    help="grouped: time only the default plan (skip the tile-assignment search)")
+                    help="grouped: time only the default plan (skip the grouped autotuner)")
+    ap.add_argument("--grouped-modes", help="grouped: comma-separated modes (default all four)")
     args = ap.parse_args(argv)
 
     import torch

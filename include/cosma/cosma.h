@@ -193,8 +193,21 @@ int cosma_sgemm_bf16x3_mnk(int m, int n, int k, float alpha, const float *A, int
  * macros of config.h (THREADBLOCK_TILE_*, WARP_TILE_*, LOAD_K). */
 int cosma_num_tile_configs(int mode);
 int cosma_tile_config(int mode, int cfg, int *out8);
-/* The (tile_cfg, split_k) the library heuristic picks for a shape. */
+/* The (tile_cfg, split_k) the library heuristic picks for a shape on the
+ * current device (its compute-unit count; without a device, a whole MI355X:
+ * 256 CUs). */
 int cosma_heuristic(int mode, int m, int n, int k, int *tile_cfg, int *split_k);
+/* The same for op(A) stored transposed or not (trans_a) on a device of `cus`
+ * compute units.  The reference's generator takes the SM count, shared
+ * memory and registers of the target as inputs
+ * (ScheduleGenerator/ParametersV100.h:27-36); here only the CU count enters
+ * the choice. */
+int cosma_heuristic_topo(int mode, int m, int n, int k, int trans_a, int cus, int *tile_cfg,
+                         int *split_k);
+/* Compute units, XCDs and peak engine clock (kHz) of the current device,
+ * queried once per device.  Returns COSMA_STATUS_NOT_SUPPORTED (and the
+ * nominal MI355X figures 256, 8, 2400000) when no device answers. */
+int cosma_device_topology(int *cus, int *xcds, int *clock_khz);
 
 /* FP64 residual statistics computed on the GPU with a deterministic
  * reduction.  out5: [0] sum (C-R)^2, [1] sum R^2, [2] max |C-R|,
@@ -216,26 +229,36 @@ int cosma_convert_f32_f64(int rows, int cols, const float *X, int64_t ldx, doubl
  * (ScheduleGenerator/ScheduleGeneratorv4.h:13-391, main.cpp:22-342) with
  * measurement.  Candidates are every built tile configuration x split-K
  * factor x XCD group size that fits the shape; each is timed with HIP events
- * on the descriptor's stream and reported with its occupancy and MFMA
- * utilisation.  The fastest is the choice. ---- */
+ * on the descriptor's stream (caches flushed before every timed call) and
+ * reported with its occupancy and achieved fraction of the mode's peak.  The
+ * fastest is the choice.  MFMA busy and L2 hit rates are hardware counters,
+ * read by the Python autotuner (cucosma_amd.autotune.counters) under
+ * rocprofv3 for chosen candidates. ---- */
 typedef struct cosma_tune_result {
     int tile_cfg, split_k, group_m;
-    int status;        /* 0, or the status of the failed launch */
-    double ms;         /* mean time per GEMM (HIP events over reps calls) */
-    double tflops;     /* 2*M*N*K / ms */
-    double mfma_util;  /* tflops / the mode's dense MFMA peak (FP32 157.3, FP64 78.6,
-                          BF16X3 2516.6/3, BF16X4 2516.6/4 TFLOP/s) */
-    int blocks;        /* workgroups of the main kernel */
-    int occupancy;     /* workgroups per CU the kernel can hold */
+    int status;           /* 0, or the status of the failed launch */
+    double ms;            /* mean time per GEMM (HIP events, one pair per timed call) */
+    double tflops;        /* 2*M*N*K / ms */
+    double achieved_frac; /* tflops / the mode's dense MFMA peak on this device (CUs x clock x
+                             per-CU rate: FP32 256, FP64 128, BF16 4096 flops per clock;
+                             BF16X3 / BF16X4 a third / a quarter of the BF16 rate) */
+    int blocks;           /* workgroups of the main kernel */
+    int occupancy;        /* workgroups per CU the kernel can hold */
     int waves_per_simd;
     size_t workspace_bytes;
 } cosma_tune_result;
 
 /* Time the candidates for the GEMM *d (its operands are used and C is
  * overwritten; d->tile_cfg / split_k / group_m >= 0 pin that parameter).
- * Fills up to max_results entries, sets *n_results and *best (index of the
- * fastest successful candidate, -1 if none).  Allocates split-K workspace as
- * needed: do not call while capturing. */
+ * warmup untimed calls, then reps timed calls per candidate, each after
+ * overwriting flush_bytes of device memory (0: no flush).  Fills up to
+ * max_results entries sorted fastest first (failed candidates last), sets
+ * *n_results and *best (0 when a candidate succeeded, else -1).  Allocates
+ * split-K workspace and the flush buffer as needed: do not call while
+ * capturing. */
+int cosma_autotune_ex(const cosma_gemm_desc *d, int warmup, int reps, size_t flush_bytes,
+                      cosma_tune_result *results, int max_results, int *n_results, int *best);
+/* cosma_autotune_ex with a 512 MiB flush (L2 and the 256 MB Infinity Cache). */
 int cosma_autotune(const cosma_gemm_desc *d, int warmup, int reps, cosma_tune_result *results,
                    int max_results, int *n_results, int *best);
 /* Tuned table consulted by every entry point whose descriptor leaves tile_cfg
@@ -275,17 +298,26 @@ typedef struct cosma_group_problem {
 
 typedef struct cosma_grouped_plan_s *cosma_grouped_plan;
 
-/* Validate the problems (same contract as cosma_gemm_ex, row-major only),
- * bucket them by tile configuration (tile_cfgs[i] for problem i; problems
- * without one -- tile_cfgs NULL or tile_cfgs[i] < 0 -- share one default
- * configuration chosen from the batch's flop-weighted output sizes), and
- * upload one device table and workgroup schedule (longest tiles first, to
- * the least-loaded workgroup) per bucket.  Allocates and synchronises; do
- * not call while capturing.  blocks_per_cu <= 0 uses the kernel's
- * occupancy. */
+/* Validate the problems (same contract as cosma_gemm_ex, row-major only;
+ * alpha == 0 or K == 0 never reads A or B and gives C = beta * C), choose a
+ * tile configuration and a split-K factor per problem, and upload, per tile
+ * family (see cosma_group_families), one device table, one workgroup
+ * schedule (longest tiles first, to the least-loaded workgroup) and the
+ * split members' partial buffer.  Problem i takes tile_cfgs[i] and
+ * splits[i] when given (tile_cfgs NULL or tile_cfgs[i] < 0, splits NULL or
+ * splits[i] <= 0: the library's choice -- each problem on the variant of the
+ * mode's first family with the least weighted padding, or one small tile for
+ * batches of small outputs; K split only for tiles that would be long poles).
+ * Allocates and synchronises; do not call while capturing.  blocks_per_cu <=
+ * 0 uses the kernel's occupancy. */
+int cosma_grouped_plan_create_ex(int mode, int count, const cosma_group_problem *problems,
+                                 const int *tile_cfgs, const int *splits, int blocks_per_cu,
+                                 cosma_grouped_plan *plan);
+/* cosma_grouped_plan_create_ex with the library's split choice. */
 int cosma_grouped_plan_create(int mode, int count, const cosma_group_problem *problems,
                               const int *tile_cfgs, int blocks_per_cu, cosma_grouped_plan *plan);
-/* One persistent-kernel launch per bucket on stream (no allocation, no copy:
+/* One persistent-kernel launch per tile family on stream, each followed by
+ * the fixed-order reduction of its split members (no allocation, no copy:
  * capturable). */
 int cosma_grouped_plan_run(cosma_grouped_plan plan, void *stream);
 /* Launch the plan as a hipGraph: captured once on first use, then
@@ -293,10 +325,42 @@ int cosma_grouped_plan_run(cosma_grouped_plan plan, void *stream);
 int cosma_grouped_plan_run_graph(cosma_grouped_plan plan, void *stream);
 /* buckets, total tiles, sum of 2*M*N*K over the problems. */
 int cosma_grouped_plan_info(cosma_grouped_plan plan, int *buckets, int *total_tiles, double *flops);
-/* Bucket b: tile config, problem count, tiles, persistent workgroups. */
+/* Bucket (tile family) b: the family's first tile config, problem count,
+ * tiles over all K slices, persistent workgroups. */
 int cosma_grouped_plan_bucket(cosma_grouped_plan plan, int b, int *tile_cfg, int *count,
                               int *tiles, int *blocks);
+/* Tile configuration and split-K factor problem i runs with. */
+int cosma_grouped_plan_choice(cosma_grouped_plan plan, int i, int *tile_cfg, int *split_k);
 int cosma_grouped_plan_destroy(cosma_grouped_plan plan);
+/* Grouped tile families of a mode: the tile configurations one persistent
+ * kernel runs side by side (one per problem).  fam < 0 returns the number of
+ * families; otherwise cfgs4 receives family fam's configurations (-1 past
+ * the last) and the return value is their number.  Every tile configuration
+ * except the pre-split ones belongs to exactly one family. */
+int cosma_group_families(int mode, int fam, int *cfgs4);
+
+/* Grouped autotuner: starting from the library's choices, the problems are
+ * put in size buckets (default tile configuration x K class: K <= 256,
+ * <= 1024, above); bucket by bucket, in decreasing order of flops, every
+ * configuration of the mode's first tile family (and the bucket's default)
+ * with K split 1, 2 or 4 ways is timed as a whole plan (graph replays, HIP
+ * events, warmup + reps on the null stream), in pairs with the plan of the
+ * choices kept so far (kept, candidate, candidate, kept), and kept when both
+ * candidate timings beat their neighbours by at least 1%.
+ * tile_cfgs / splits (count entries) receive the choices, for
+ * cosma_grouped_plan_create_ex.  Every timed plan runs on the problems'
+ * buffers (C is overwritten).  results (up to max_results) list the timed
+ * candidates in order: bucket -1 is the starting plan. */
+typedef struct cosma_group_tune_result {
+    int bucket;   /* size bucket (default tile config * 3 + K class), -1: the starting plan */
+    int tile_cfg; /* configuration tried for the bucket */
+    int split_k;  /* split tried for the bucket */
+    int status;
+    double ms;    /* time per pass of the whole plan (mean of the candidate's two timings) */
+} cosma_group_tune_result;
+int cosma_grouped_autotune(int mode, int count, const cosma_group_problem *problems, int warmup,
+                           int reps, int *tile_cfgs, int *splits, cosma_group_tune_result *results,
+                           int max_results, int *n_results);
 
 /* Human-readable text for a status code (static storage). */
 const char *cosma_status_string(int status);

@@ -206,3 +206,121 @@ int call(const float* A, const float* B, float* Cp) { return cosmaSgemm(A, K, B,
         open(p, "w").write(src)
         subprocess.run(["gcc", "-c", "-I", os.path.join(ROOT, "include"), p, "-o",
                         os.path.join(td, "t.o")], check=True)
+
+
+def test_heuristic_empty_outputs_return_at_once():
+    """m == 0 or n == 0 returns a choice without looping or dividing by the
+    empty tile count, for every mode and a deep K."""
+    import cucosma_amd
+
+    for mode in range(4):
+        for m, n, k in ((0, 64, 64), (64, 0, 64), (0, 0, 0), (0, 64, 1 << 16), (64, 0, 1 << 20)):
+            cfg, split = cucosma_amd.heuristic(mode, m, n, k)
+            assert 0 <= cfg < cucosma_amd.lib.cosma_num_tile_configs(mode) and split == 1
+
+
+def test_heuristic_follows_the_cu_count():
+    """The heuristic takes the CU count as an input: on a 128-CU device (half
+    an MI355X) the deep-K splits fill 128 CUs, large outputs keep the large
+    tiles, and the round-counting rules count rounds of 128."""
+    import cucosma_amd
+
+    h = cucosma_amd.heuristic
+    tc = cucosma_amd.tile_configs
+    for cus in (256, 128, 64):
+        for mode in (0, 2, 3):
+            cfg, split = h(mode, 256, 256, 1 << 20, cus=cus)
+            t = tc(mode)[cfg]
+            assert (256 // t["BM"]) * (256 // t["BN"]) * split == cus, (mode, cus)
+    # Nominal topology is the default without a device.
+    assert h(2, 8192, 8192, 8192, cus=256) == h(2, 8192, 8192, 8192)
+    assert h(2, 8192, 8192, 8192, cus=128) == (8, 1)
+    # FP64 large outputs: split in two when the last round of 2 * cus is at most a quarter full.
+    assert h(1, 3072, 3072, 3072, cus=256) == (7, 2)   # 576 tiles: last round 64 of 512
+    assert h(1, 3072, 3072, 3072, cus=128) == (7, 2)   # last round 64 of 256
+    assert h(1, 2304, 2304, 2048, cus=128) == (7, 1)   # 324 tiles: last round 68 of 256
+    # 400 128x128 tiles are "large" (two rounds of workgroups) only on 128 CUs.
+    assert h(1, 2560, 2560, 2048, cus=256)[0] == 3
+    assert h(1, 2560, 2560, 2048, cus=128) == (7, 1)
+    # The medium FP64 rule counts slots of the given CUs (1024^3: 256 64x64 tiles).
+    assert h(1, 1024, 1024, 1024, cus=256) == (1, 3)
+    assert h(1, 1024, 1024, 1024, cus=64)[1] == 1
+    lib = cucosma_amd.lib
+    out = (C.c_int(), C.c_int())
+    assert lib.cosma_heuristic_topo(0, 64, 64, 64, 0, 0, C.byref(out[0]), C.byref(out[1])) == -1
+    assert lib.cosma_heuristic_topo(0, 64, 64, 64, 2, 256, C.byref(out[0]), C.byref(out[1])) == -1
+
+
+def test_device_topology_without_a_device_is_nominal():
+    import cucosma_amd
+    import torch
+
+    t = cucosma_amd.device_topology()
+    if torch.cuda.is_available():
+        assert t["queried"] and t["cus"] > 0 and t["xcds"] >= 1
+    else:
+        assert not t["queried"] and (t["cus"], t["xcds"], t["clock_khz"]) == (256, 8, 2400000)
+        assert abs(cucosma_amd.peak_tflops("bf16", t) - 2516.58) < 0.1
+        assert abs(cucosma_amd.peak_tflops("fp32", t) - 157.29) < 0.01
+        assert abs(cucosma_amd.peak_tflops("fp64", t) - 78.64) < 0.01
+        assert abs(cucosma_amd.peak_tflops("bf16x3", t) - 838.86) < 0.01
+
+
+def test_group_m_and_workspace_checks():
+    import cucosma_amd
+
+    lib = cucosma_amd.lib
+    for kw, status in ((dict(group_m=70000), -1), (dict(workspace=0x5008, workspace_bytes=1 << 20), -2)):
+        d = _desc(**kw)
+        assert lib.cosma_gemm_ex(C.byref(d)) == status
+    assert lib.cosma_workspace_set(0x5004, 4096) == -2
+    assert lib.cosma_tune_set(0, 64, 64, 64, 0, 0, 0, 1, 1 << 20) == -1
+
+
+def test_every_tile_configuration_in_one_group_family():
+    """Each non-pre-split tile configuration belongs to exactly one grouped
+    family; a family's configurations share one workgroup size; the first
+    family of each mode holds its large tile and that tile's thin variants."""
+    import cucosma_amd
+    from cucosma_amd._lib import MODES
+
+    for name, mode in MODES.items():
+        fams = cucosma_amd.group_families(name)
+        tiles = cucosma_amd.tile_configs(mode)
+        seen = [c for f in fams for c in f]
+        assert sorted(seen) == [t["cfg"] for t in tiles if t["pipe"] != 5], name
+        for f in fams:
+            assert len({tiles[c]["threads"] for c in f}) == 1, (name, f)
+        big = tiles[fams[0][0]]
+        assert len(fams[0]) >= 3
+        for c in fams[0][1:]:
+            assert tiles[c]["BM"] * tiles[c]["BN"] <= big["BM"] * big["BN"]
+            assert (tiles[c]["BM"], tiles[c]["BN"]) != (big["BM"], big["BN"])
+
+
+def test_grouped_plan_rejects_bad_problems_on_host():
+    import cucosma_amd
+    from cucosma_amd._lib import GroupProblem
+
+    lib = cucosma_amd.lib
+    plan = C.c_void_p()
+    bad = (GroupProblem * 1)(GroupProblem(-1, 4, 4, 0x1000, 4, 0x1000, 4, 0x1000, 4, 1.0, 0.0))
+    assert lib.cosma_grouped_plan_create(0, 1, bad, None, 0, C.byref(plan)) == -1
+    bad = (GroupProblem * 1)(GroupProblem(4, 4, 4, 0x1000, 3, 0x1000, 4, 0x1000, 4, 1.0, 0.0))
+    assert lib.cosma_grouped_plan_create(0, 1, bad, None, 0, C.byref(plan)) == -1
+    ok = (GroupProblem * 1)(GroupProblem(4, 4, 4, 0x1000, 4, 0x1000, 4, 0x1000, 4, 1.0, 0.0))
+    assert lib.cosma_grouped_plan_create(2, 1, ok, (C.c_int * 1)(8), 0, C.byref(plan)) == -3  # pre-split
+    assert lib.cosma_grouped_plan_create_ex(0, 1, ok, None, (C.c_int * 1)(5000), 0, C.byref(plan)) == -1
+
+
+def test_torch_wrappers_reject_host_tensors():
+    import cucosma_amd
+    import torch
+
+    A = torch.zeros(8, 8)
+    with pytest.raises(ValueError):
+        cucosma_amd.gemm(A, A)
+    with pytest.raises(ValueError):
+        cucosma_amd.GroupedGemm([A], [A], [A])
+    with pytest.raises(TypeError):
+        cucosma_amd.GroupedGemm([A], [A], [A], mode="fp64")

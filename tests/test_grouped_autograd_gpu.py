@@ -92,14 +92,16 @@ def test_grouped_empty_and_explicit_tiles(cc, dev):
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16x4", "fp64"])
 def test_grouped_default_plan_is_one_launch(cc, dev, mode):
-    """Without tile_cfgs the whole batch shares one tile configuration (one
-    persistent launch): the mode's large tile when outputs with a side of at
-    least 256 hold half the flops, else 128x128, else 64x64."""
+    """Without tile_cfgs the whole batch is one persistent launch: the mode's
+    first tile family (the large tile and its thin variants) when outputs
+    with a side of at least 256 hold half the flops, else 128x128, else
+    64x64."""
     from cucosma_amd.harness import grouped_sizes
 
     dt = torch.float64 if mode == "fp64" else torch.float32
-    # the large tiles: FP32 256x128 (16 waves), FP64 128x128 (8 waves), BF16 256x256 on the fly
-    large = {"fp32": 7, "fp64": 7, "bf16x3": 6, "bf16x4": 3}[mode]
+    # the first tile family: FP32 256x128 (16 waves), FP64 128x128 (8 waves),
+    # BF16 256x256 on the fly, each with its thin and small variants
+    large = cc.group_families(mode)[0][0]
     for sizes, want in ((grouped_sizes(), large), ([(96, 160, 64)] * 3 + [(40, 32, 512)], 0),
                         ([(64, 64, 64), (32, 96, 128)], 1)):
         As = [torch.empty((m, k), device=dev, dtype=dt) for m, n, k in sizes]
@@ -109,6 +111,8 @@ def test_grouped_default_plan_is_one_launch(cc, dev, mode):
         b = plan.info()["buckets"]
         assert len(b) == 1 and b[0]["tile_cfg"] == want and b[0]["problems"] == len(sizes), b
         assert 1 <= b[0]["blocks"] <= b[0]["tiles"]
+        if want == large:
+            assert {c for c, s in plan.choices()} <= set(cc.group_families(mode)[0])
         plan.close()
         del As, Bs, Cs
 
@@ -129,8 +133,9 @@ def test_grouped_bitwise_equals_single_launches(cc, dev, mode):
     Bs = [(torch.rand(k, n, generator=g, dtype=dt) * 2 - 1).to(dev) for m, n, k in sizes]
     C0s = [(torch.rand(m, n, generator=g, dtype=dt) * 2 - 1).to(dev) for m, n, k in sizes]
     Cs = [c.clone() for c in C0s]
-    plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, alpha=1.5, beta=0.5, tile_cfgs=cfgs)
-    assert {b["tile_cfg"] for b in plan.info()["buckets"]} == set(usable)
+    plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, alpha=1.5, beta=0.5, tile_cfgs=cfgs, splits=[1] * len(cfgs))
+    assert [c for c, s in plan.choices()] == cfgs
+    assert len(plan.info()["buckets"]) == len(cc.group_families(mode))
     plan.run()
     for A, B, C0, Cg, cfg in zip(As, Bs, C0s, Cs, cfgs):
         Cs1 = C0.clone()

@@ -67,7 +67,8 @@ def test_autotune_picks_a_valid_fastest(dev):
     rec = autotune.tune("fp32", 512, 384, 2048, warmup=1, reps=2)
     ok = [c for c in rec["candidates"] if c["status"] == 0]
     assert ok and rec["best"] is not None
-    assert rec["best"]["ms"] == min(c["ms"] for c in ok)
+    assert rec["best"]["ms"] == min(c["ms"] for c in ok) == rec["candidates"][0]["ms"]
+    assert rec["flushed"] and all(0 < c["achieved_frac"] <= 1 for c in ok)
     assert all(c["occupancy"] >= 1 for c in ok)
     assert any(c["split_k"] > 1 for c in ok)
 
@@ -91,3 +92,20 @@ def test_benchmark_protocol_and_tuned_dispatch(dev):
     C2 = cc.gemm(A, B)
     assert cc.lib.cosma_tune_clear() == 0
     assert torch.allclose(C2.double(), ref, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.gpu
+def test_autotune_record_carries_mfma_busy_and_l2_hit(dev):
+    """The tuned record's best candidate carries counters measured by
+    rocprofv3: MFMA busy share and L2 hit rate, each in (0, 1]."""
+    import shutil
+
+    from cucosma_amd import autotune
+
+    if shutil.which("rocprofv3") is None:
+        pytest.skip("rocprofv3 missing")
+    rec = autotune.tune("bf16x3", 2048, 2048, 2048, warmup=1, reps=2, with_counters=True)
+    b = rec["best"]
+    assert 0.0 < b["mfma_busy"] <= 1.0, b["counters"]
+    assert 0.0 < b["l2_hit"] <= 1.0, b["counters"]
+    assert b["counters"]["fetch_kib"] > 0
