@@ -16,6 +16,7 @@ STATUS_INVALID_VALUE = -1
 STATUS_MISALIGNED = -2
 STATUS_NOT_SUPPORTED = -3
 STATUS_WORKSPACE = -4
+STATUS_SCRATCH = -5
 
 MODE_FP32 = 0
 MODE_FP64 = 1
@@ -118,6 +119,8 @@ def _load():
     sig = {
         "cosma_gemm_desc_init": (None, [C.POINTER(GemmDesc)]),
         "cosma_gemm_ex": (i, [C.POINTER(GemmDesc)]),
+        "cosma_gemm_ex_static": (i, [C.POINTER(GemmDesc)]),
+        "cosma_static_scratch_bytes": (C.c_size_t, []),
         "cosma_workspace_query": (i, [C.POINTER(GemmDesc), C.POINTER(C.c_size_t)]),
         "cosma_workspace_set": (i, [vp, C.c_size_t]),
         "cosma_workspace_release": (i, [C.POINTER(C.c_size_t)]),

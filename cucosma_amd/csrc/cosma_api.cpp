@@ -6,7 +6,7 @@
 // reference), split-K workspace (:2824-2838) and the launches.  Every check
 // that can fail runs before the first HIP call, so a contract violation
 // returns a negative status without touching the GPU.
-#include <hip/hip_runtime_api.h>
+#include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
@@ -474,9 +474,46 @@ void resolve_choice(const cosma_gemm_desc* d, int* cfg, int* split, int* group_m
   if (*split > d->k) *split = d->k > 0 ? d->k : 1;
 }
 
-int run(const cosma_gemm_desc* d) {
+// ---- Static scratch for callers that cannot allocate ----------------------
+// cosma_gemm_ex_static keeps split-K partials and the planes of operands
+// split inside the call in this file-scope device array instead of a
+// workspace.  Each GPU the library runs on holds its own copy until the
+// process ends.  The array's host-side name is not a device address: its
+// address on a device is taken once per device (scratch_ptr) with
+// hipGetSymbolAddress, and only that address is passed to kernels.
+constexpr size_t kScratchBytes = (size_t)64 << 20;  // 256x256 outputs with 256 FP32 K slices
+static __device__ __align__(256) char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;  // per device: scratch's address there
+
+int scratch_ptr(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!ptr) return (int)hipErrorInvalidSymbol;
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+int run_impl(const cosma_gemm_desc* d, bool use_scratch);
+int run(const cosma_gemm_desc* d) { return run_impl(d, false); }
+
+// use_scratch: every buffer beyond the operands comes from the static
+// scratch array (no allocation); the pre-split configuration gives way to
+// the on-the-fly tile with the same products in the same order when its
+// planes do not fit.
+int run_impl(const cosma_gemm_desc* d, bool use_scratch) {
   int st = validate(d);
   if (st != 0) return st;
+  if (use_scratch && d->workspace) return COSMA_STATUS_INVALID_VALUE;
   if (d->m == 0 || d->n == 0) return 0;
   hipStream_t s = static_cast<hipStream_t>(d->stream);
   const bool uses_ab = d->k > 0 && d->alpha != 0.0;
@@ -517,10 +554,17 @@ int run(const cosma_gemm_desc* d) {
   p.b_planes = d->b_planes;
   p.stream = s;
   size_t planes_off = 0;
-  const size_t need = workspace_need(d, cfg, split, &planes_off);
+  size_t need = workspace_need(d, cfg, split, &planes_off);
+  if (use_scratch && need > kScratchBytes && is_planar(d->mode, cfg) && d->tile_cfg < 0 && !d->a_planes &&
+      !d->b_planes) {
+    cfg = large_cfg(d->mode);  // on-the-fly 256x256 tile: bitwise the pre-split result
+    p.cfg = cfg;
+    need = workspace_need(d, cfg, split, &planes_off);
+  }
+  if (use_scratch && need > kScratchBytes) return COSMA_STATUS_SCRATCH;
   if (need > 0) {
     void* ws = nullptr;
-    st = resolve_workspace(d, need, s, &ws);
+    st = use_scratch ? scratch_ptr(&ws) : resolve_workspace(d, need, s, &ws);
     if (st != 0) return st;
     if (split > 1) p.ws = ws;
     if (is_planar(d->mode, cfg)) p.planes = static_cast<char*>(ws) + planes_off;
@@ -1258,6 +1302,10 @@ void cosma_gemm_desc_init(cosma_gemm_desc* d) {
 
 int cosma_gemm_ex(const cosma_gemm_desc* d) { return run(d); }
 
+int cosma_gemm_ex_static(const cosma_gemm_desc* d) { return run_impl(d, true); }
+
+size_t cosma_static_scratch_bytes(void) { return kScratchBytes; }
+
 int cosma_workspace_query(const cosma_gemm_desc* d, size_t* bytes) {
   int st = validate(d);
   if (st != 0) return st;
@@ -1662,6 +1710,7 @@ const char* cosma_status_string(int status) {
     case COSMA_STATUS_MISALIGNED: return "operand base not aligned to its element size";
     case COSMA_STATUS_NOT_SUPPORTED: return "combination not supported";
     case COSMA_STATUS_WORKSPACE: return "split-K workspace unavailable while the stream is capturing";
+    case COSMA_STATUS_SCRATCH: return "the call needs more than the static scratch array holds";
     default: return status > 0 ? hipGetErrorString((hipError_t)status) : "unknown status";
   }
 }

@@ -53,6 +53,7 @@ extern "C" {
 #define COSMA_STATUS_MISALIGNED (-2)    /* operand base not aligned to its element size */
 #define COSMA_STATUS_NOT_SUPPORTED (-3) /* combination not built (e.g. both operands transposed) */
 #define COSMA_STATUS_WORKSPACE (-4)     /* split-K workspace missing while the stream is capturing */
+#define COSMA_STATUS_SCRATCH (-5)       /* cosma_gemm_ex_static: needs more than the static scratch */
 
 #define COSMA_MODE_FP32 0   /* FP32 in, FP32 MFMA, FP32 out */
 #define COSMA_MODE_FP64 1   /* FP64 in, FP64 MFMA, FP64 out */
@@ -103,6 +104,24 @@ void cosma_gemm_desc_init(cosma_gemm_desc *d);
 /* Run the GEMM described by *d.  Replaces the body of cosmaSgemm
  * (Kernel/cuda/src/cuCOSMAV100.cuh:2774-2848) for every mode and layout. */
 int cosma_gemm_ex(const cosma_gemm_desc *d);
+
+/* cosma_gemm_ex for callers that cannot allocate (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  Allocates nothing: split-K partials and the planes of
+ * operands split inside the call live in a static device array of
+ * cosma_static_scratch_bytes() bytes (64 MiB) per GPU, held until the
+ * process ends; its address on a device is looked up on the first call on
+ * that device.  d->workspace must be NULL (an installed workspace is not
+ * used).  Calls that need more return COSMA_STATUS_SCRATCH, except that the
+ * library's pre-split BF16 configuration gives way to the on-the-fly tile,
+ * which computes the same products in the same order (bitwise the same C).
+ * Nothing in the array has to start at zero: every call writes all it
+ * reads.  All calls on one GPU share its array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be ordered by the
+ * caller. */
+int cosma_gemm_ex_static(const cosma_gemm_desc *d);
+/* Size of the static scratch array of cosma_gemm_ex_static. */
+size_t cosma_static_scratch_bytes(void);
 
 /* Bytes of workspace the call *d needs: split-K partials, then the planes of
  * operands split inside the call (0 when neither applies). */
