@@ -59,11 +59,35 @@ def time_launches(fn, stream, steps: int, warmup: int, torch):
     return wall, float(np.mean(per)), float(np.min(per))
 
 
+def cpu_share() -> int:
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when set (the GPU boxes give one GPU a share of 16)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(A: np.ndarray, B: np.ndarray, target_s: float = 12.0) -> dict:
-    """Time the reference's CPU verification product on a row sample."""
+    """Time the reference's CPU verification product (Matrix<T>::operator*,
+    naive i-j-k with a float accumulator): on a row sample of this workload
+    over the process's CPU share, and config 1 (1024^3) in full on one core
+    and on the CPU share."""
     import oracle  # the CPU restatement; used here only as the timed baseline
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_share()
     rows = threads  # one row per thread per round; grow until the sample takes ~target_s
     while True:
         t0 = time.perf_counter()
@@ -77,6 +101,15 @@ def cpu_baseline(A: np.ndarray, B: np.ndarray, target_s: float = 12.0) -> dict:
     t0 = time.perf_counter()
     oracle.cpu_naive_sgemm(A, B, rows=1, row0=rows, nthreads=1)
     dt1 = time.perf_counter() - t0
+    # Config 1 in full: 1024^3, the seed-0 stream of the plain-SGEMM workload.
+    A1, B1, _ = oracle.workload_plain_1024()
+    f1 = 2.0 * 1024**3
+    t0 = time.perf_counter()
+    oracle.cpu_naive_sgemm(A1, B1, nthreads=1)
+    c1_one = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.cpu_naive_sgemm(A1, B1, nthreads=threads)
+    c1_all = time.perf_counter() - t0
     return {
         "value": flops / dt / 1e12,
         "unit": "TFLOP/s",
@@ -86,6 +119,13 @@ def cpu_baseline(A: np.ndarray, B: np.ndarray, target_s: float = 12.0) -> dict:
                    f"accumulator, OpenMP over rows, {threads} threads) in {dt:.1f} s; "
                    f"1 row on 1 core: {2.0 * N * K / dt1 / 1e9:.3f} GFLOP/s; extrapolated"),
         "single_core_value": 2.0 * N * K / dt1 / 1e12,
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "cpu_share_basis": "process affinity capped by OMP_NUM_THREADS (16 threads per GPU on the GPU boxes)",
+        "config1_1024cubed_full": {
+            "one_core_s": round(c1_one, 3), "one_core_gflops": round(f1 / c1_one / 1e9, 3),
+            "all_cores_s": round(c1_all, 3), "all_cores_gflops": round(f1 / c1_all / 1e9, 3),
+            "cores": threads},
     }
 
 
@@ -160,6 +200,10 @@ def main() -> int:
     wall_4, ev_4, min_4 = time_launches(run_bf16x4, stream, args.steps, args.warmup, torch)
     log(f"bf16x4: {ev_4:.3f} ms/launch ({flops / ev_4 / 1e9:.1f} TFLOP/s)")
 
+    topo = cc.device_topology()
+    peak_dev = cc.peak_tflops("bf16x3", topo)
+    peak_basis_dev = (f"{topo['cus']} CUs x {topo['clock_khz'] / 1e3:.0f} MHz (device peak clock) x 4096 bf16 "
+                      f"flops/clock/CU / 3 products = {peak_dev:.1f} TFLOP/s")
     cfg_b, split_b = cc.heuristic(cc.MODE_BF16X3, M, N, K)
     tile_b = cc.tile_configs(cc.MODE_BF16X3)[cfg_b]
     presplit = tile_b["pipe"] == 5
@@ -223,7 +267,9 @@ def main() -> int:
                      "peak": round(PEAK_BF16X3, 1), "unit": "TFLOP/s",
                      "frac": round(flops / ev_k / 1e9 / PEAK_BF16X3, 4), "traffic": traffic,
                      "kernel": kname,
-                     "peak_basis": "MI355X dense bf16 MFMA 2516.6 TF / 3 products"},
+                     "peak_basis": "MI355X dense bf16 MFMA 2516.6 TF / 3 products",
+                     "peak_device": round(peak_dev, 1), "frac_device": round(flops / ev_k / 1e9 / peak_dev, 4),
+                     "peak_device_basis": peak_basis_dev},
     }
 
     if not args.no_check:
@@ -243,6 +289,8 @@ def main() -> int:
         result["bf16x3"]["max_rel_err"] = eb["max_rel"]
         result["plain_fp32"]["normwise_err"] = ep["normwise"]
         result["plain_fp32"]["max_rel_err"] = ep["max_rel"]
+        result["max_rel_err_basis"] = ("max |C - R| / |R| over the entries with |R| >= 1e-3 max|R| "
+                                       "(R: the GPU FP64 product)")
         result["err_ratio_bf16x3_vs_fp32"] = eb["normwise"] / ep["normwise"] if ep["normwise"] else None
         result["fp64_reference_spotcheck_rel"] = spot
         del R

@@ -144,6 +144,8 @@ def _load():
         "cosma_device_topology": (i, [C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "cosma_error_stats_f32": (i, [i, i, vp, i64, vp, i64, f64p, vp]),
         "cosma_error_stats_f64": (i, [i, i, vp, i64, vp, i64, f64p, vp]),
+        "cosma_error_stats_ex_f32": (i, [i, i, vp, i64, vp, i64, d, f64p, vp]),
+        "cosma_error_stats_ex_f64": (i, [i, i, vp, i64, vp, i64, d, f64p, vp]),
         "cosma_sumsq_f32": (i, [i, i, vp, i64, f64p, vp]),
         "cosma_sumsq_f64": (i, [i, i, vp, i64, f64p, vp]),
         "cosma_convert_f32_f64": (i, [i, i, vp, i64, vp, i64, vp]),

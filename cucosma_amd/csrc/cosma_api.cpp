@@ -1474,27 +1474,37 @@ int cosma_device_topology(int* cus, int* xcds, int* clock_khz) {
   return t.queried ? 0 : COSMA_STATUS_NOT_SUPPORTED;
 }
 
-int cosma_error_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
-                          int64_t ldr, double* out5, void* stream) {
-  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols)
+int cosma_error_stats_ex_f32(int rows, int cols, const float* C, int64_t ldc, const double* R, int64_t ldr,
+                             double rel_floor, double* out5, void* stream) {
+  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols ||
+      !(rel_floor >= 0.0))
     return COSMA_STATUS_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(g_stats_mu);
   double *part, *dout;
   int st = stats_scratch(&part, &dout);
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return finish_stats(cosma::run_err_stats_f32(rows, cols, C, ldc, R, ldr, part, dout, s), dout, out5, s);
+  return finish_stats(cosma::run_err_stats_f32(rows, cols, C, ldc, R, ldr, rel_floor, part, dout, s), dout, out5, s);
+}
+int cosma_error_stats_ex_f64(int rows, int cols, const double* C, int64_t ldc, const double* R, int64_t ldr,
+                             double rel_floor, double* out5, void* stream) {
+  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols ||
+      !(rel_floor >= 0.0))
+    return COSMA_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_stats_mu);
+  double *part, *dout;
+  int st = stats_scratch(&part, &dout);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return finish_stats(cosma::run_err_stats_f64(rows, cols, C, ldc, R, ldr, rel_floor, part, dout, s), dout, out5, s);
+}
+int cosma_error_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
+                          int64_t ldr, double* out5, void* stream) {
+  return cosma_error_stats_ex_f32(rows, cols, C, ldc, R, ldr, 0.0, out5, stream);
 }
 int cosma_error_stats_f64(int rows, int cols, const double* C, int64_t ldc, const double* R,
                           int64_t ldr, double* out5, void* stream) {
-  if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && (!C || !R)) || ldc < cols || ldr < cols)
-    return COSMA_STATUS_INVALID_VALUE;
-  std::lock_guard<std::mutex> lk(g_stats_mu);
-  double *part, *dout;
-  int st = stats_scratch(&part, &dout);
-  if (st) return st;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  return finish_stats(cosma::run_err_stats_f64(rows, cols, C, ldc, R, ldr, part, dout, s), dout, out5, s);
+  return cosma_error_stats_ex_f64(rows, cols, C, ldc, R, ldr, 0.0, out5, stream);
 }
 int cosma_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* out5, void* stream) {
   if (rows < 0 || cols < 0 || !out5 || ((rows && cols) && !X) || ldx < cols)

@@ -103,7 +103,7 @@ __device__ __forceinline__ void block_stat_reduce(double (&mine)[kNStat], double
 template <typename TC>
 __global__ void __launch_bounds__(kStatThreads)
     err_stats_partial(int rows, int cols, const TC* __restrict__ C, int64_t ldc,
-                      const double* __restrict__ R, int64_t ldr, double* __restrict__ part) {
+                      const double* __restrict__ R, int64_t ldr, double rel_floor, double* __restrict__ part) {
   double mine[kNStat] = {0, 0, 0, 0, 0};
   const int64_t total = (int64_t)rows * cols;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(kStatThreads)
     mine[0] += d * d;
     mine[1] += r * r;
     mine[2] = fmax(mine[2], d);
-    if (r != 0.0) mine[3] = fmax(mine[3], d / fabs(r));
+    if (r != 0.0 && fabs(r) >= rel_floor) mine[3] = fmax(mine[3], d / fabs(r));
     mine[4] = fmax(mine[4], fabs(r));
   }
   block_stat_reduce(mine, part + (int64_t)blockIdx.x * kNStat);
@@ -289,16 +289,16 @@ hipError_t run_flush(void* buf, size_t bytes, hipStream_t s) {
 static_assert(kStatParts == kStatBlocks, "partial buffer size must match the stats grid");
 
 hipError_t run_err_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
-                             int64_t ldr, double* part, double* out, hipStream_t s) {
+                             int64_t ldr, double rel_floor, double* part, double* out, hipStream_t s) {
   hipLaunchKernelGGL(err_stats_partial<float>, dim3(kStatBlocks), dim3(kStatThreads), 0, s, rows,
-                     cols, C, ldc, R, ldr, part);
+                     cols, C, ldc, R, ldr, rel_floor, part);
   hipLaunchKernelGGL(stats_final, dim3(1), dim3(kStatThreads), 0, s, part, kStatBlocks, out);
   return hipGetLastError();
 }
 hipError_t run_err_stats_f64(int rows, int cols, const double* C, int64_t ldc, const double* R,
-                             int64_t ldr, double* part, double* out, hipStream_t s) {
+                             int64_t ldr, double rel_floor, double* part, double* out, hipStream_t s) {
   hipLaunchKernelGGL(err_stats_partial<double>, dim3(kStatBlocks), dim3(kStatThreads), 0, s, rows,
-                     cols, C, ldc, R, ldr, part);
+                     cols, C, ldc, R, ldr, rel_floor, part);
   hipLaunchKernelGGL(stats_final, dim3(1), dim3(kStatThreads), 0, s, part, kStatBlocks, out);
   return hipGetLastError();
 }

@@ -140,9 +140,9 @@ hipError_t run_flush(void* buf, size_t bytes, hipStream_t s);
 // FP64 statistics; part must hold kStatParts * 5 doubles, out 5 doubles (device).
 constexpr int kStatParts = 1024;
 hipError_t run_err_stats_f32(int rows, int cols, const float* C, int64_t ldc, const double* R,
-                             int64_t ldr, double* part, double* out, hipStream_t s);
+                             int64_t ldr, double rel_floor, double* part, double* out, hipStream_t s);
 hipError_t run_err_stats_f64(int rows, int cols, const double* C, int64_t ldc, const double* R,
-                             int64_t ldr, double* part, double* out, hipStream_t s);
+                             int64_t ldr, double rel_floor, double* part, double* out, hipStream_t s);
 hipError_t run_sumsq_f32(int rows, int cols, const float* X, int64_t ldx, double* part, double* out,
                          hipStream_t s);
 hipError_t run_sumsq_f64(int rows, int cols, const double* X, int64_t ldx, double* part,

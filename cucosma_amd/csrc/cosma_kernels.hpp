@@ -572,32 +572,126 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
   if constexpr (PIPE == 0) mid();
 }
 
+// 4x4 transpose inside each quad of lanes (lanes 4c .. 4c+3): on entry lane
+// 4c+b holds x[r] = element (row r, column b) of the quad's 4x4 block; on
+// exit it holds x[j] = element (row b, column j).  Two butterfly stages swap
+// bit 0, then bit 1, of the (lane, element) indices with DPP quad
+// permutations (xor 1: quad_perm [1,0,3,2]; xor 2: [2,3,0,1]).
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ f32x4 quad_transpose(f32x4 x, int b) {
+  const bool o0 = (b & 1) != 0;
+  float s0 = o0 ? x[0] : x[1], s1 = o0 ? x[2] : x[3];
+  float r0 = dpp_xor1(s0), r1 = dpp_xor1(s1);
+  if (o0) {
+    x[0] = r0;
+    x[2] = r1;
+  } else {
+    x[1] = r0;
+    x[3] = r1;
+  }
+  const bool o1 = (b & 2) != 0;
+  s0 = o1 ? x[0] : x[2];
+  s1 = o1 ? x[1] : x[3];
+  r0 = dpp_xor2(s0);
+  r1 = dpp_xor2(s1);
+  if (o1) {
+    x[0] = r0;
+    x[1] = r1;
+  } else {
+    x[2] = r0;
+    x[3] = r1;
+  }
+  return x;
+}
+
 // Epilogue of the whole wave tile.  split == 1: C = alpha*acc, plus beta*C
 // when beta != 0 (C is not read when beta == 0); split > 1: the raw partial
 // goes to ws[z] (scale 1, exact).  Destination, leading dimension, scale and
 // whether to read C are uniform and chosen once; one loop serves all cases
 // (separate loops per case made the compiler keep extra copies of the
-// accumulators, raising the kernel's register count).  Every address is a
-// per-lane base pointer plus a uniform (16 i + row) * ld offset.  With beta
-// != 0 the C values of one sub-tile row (4 x TN per lane) are loaded together
-// before any of them is stored: a load cannot move above an earlier store
-// that may alias it (ld is a run-time value), so loading each element just
-// before its own store paid one memory latency per element.
+// accumulators, raising the kernel's register count).
+//
+// The split modes' FP32 accumulators hold 4 consecutive rows of one column
+// per lane; when the destination's rows start on 16-byte boundaries the 4x4
+// blocks of each lane quad are transposed first (quad_transpose), so every
+// lane owns 4 consecutive columns of one row and C is read and written with
+// 16-byte accesses -- a quarter of the memory instructions of element-wise
+// stores (a 256x256 tile's stores cost a grouped launch 4-7 us per tile
+// element-wise; with 16-byte stores the 512-problem BF16x3 batch ran 5%
+// faster, the pre-split BF16x4 8192^3 kernel 3%).  Each element's arithmetic
+// is unchanged, so the values are bitwise those of the element-wise path,
+// which serves unaligned destinations, FP64, and FP32 (whose 16-wave tile
+// sits at 128 registers: the transpose made it spill, and its grouped launch
+// ran 3% slower).  With beta != 0 the C values of one sub-tile row
+// are loaded together before any of them is stored: a load cannot move above
+// an earlier store that may alias it (ld is a run-time value).
 template <int MODE, bool EDGE, int TM, int TN>
 __device__ __forceinline__ void epilogue(const GemmParams<MODE>& p, int z, int row0, int col0,
                                          int lane, const typename Traits<MODE>::Acc (&acc)[TM][TN]) {
   using E = typename Traits<MODE>::E;
   const int g = lane >> 4;
-  const int lrow = row0 + (MODE == kF64 ? g : 4 * g);  // row of element 0 of this lane
-  const int lcol = col0 + (lane & 15);
   const bool partial = p.split > 1;
   const int64_t ld = partial ? (int64_t)p.N : p.ldc;
-  gptr<E> base = to_global(partial ? p.ws + ((int64_t)z * p.M + lrow) * p.N + lcol
-                                    : p.C + (int64_t)lrow * p.ldc + lcol);
+  E* const dst = partial ? p.ws + (int64_t)z * p.M * p.N : p.C;
   const E scale = partial ? E(1) : p.alpha;
   const bool read_c = !partial && p.beta != E(0);
   const int act = partial ? kActNone : p.act;
   const E beta = p.beta;
+  if constexpr (is_split(MODE)) {
+    if (((uintptr_t)dst % 16 == 0) && (ld % 4 == 0)) {
+      const int b = lane & 3;
+      const int trow = row0 + 4 * g + b;     // row of this lane after the transpose (sub-tile row 0)
+      const int tcol = col0 + (lane & 12);   // first of its 4 columns (sub-tile column 0)
+      gptr<E> base = to_global(dst + (int64_t)trow * ld + tcol);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int row = trow + i * 16;
+        gptr<E> q = base + (int64_t)(i * 16) * ld;
+        const bool row_ok = !EDGE || row < p.M;
+        f32x4 cv[TN];
+        if (read_c) {
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            const int col = tcol + j * 16;
+            if (!EDGE || (row_ok && col + 4 <= p.N)) {
+              cv[j] = *reinterpret_cast<gptr<const f32x4>>(q + j * 16);
+            } else {
+#pragma unroll
+              for (int e = 0; e < 4; ++e) cv[j][e] = (row_ok && col + e < p.N) ? q[j * 16 + e] : E(0);
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          f32x4 v = quad_transpose(acc[i][j], b);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            E out = scale * v[e];
+            if (read_c) out = fma_e(beta, cv[j][e], out);
+            if (act != kActNone) out = activate(out, act);
+            v[e] = out;
+          }
+          const int col = tcol + j * 16;
+          if (!EDGE || (row_ok && col + 4 <= p.N)) {
+            *reinterpret_cast<gptr<f32x4>>(q + j * 16) = v;
+          } else if (row_ok) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (col + e < p.N) q[j * 16 + e] = v[e];
+          }
+        }
+      }
+      return;
+    }
+  }
+  const int lrow = row0 + (MODE == kF64 ? g : 4 * g);  // row of element 0 of this lane
+  const int lcol = col0 + (lane & 15);
+  gptr<E> base = to_global(dst + (int64_t)lrow * ld + lcol);
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
     E cv[4][TN];
@@ -796,10 +890,10 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN)) __attribute__((amd
 // kernel, so a problem's result is bitwise the single launch's of the same
 // tile configuration and split: slices of a split member write their raw
 // partials to the member's workspace (ws[z][M][N]), which
-// grouped_reduce_kernel sums in ascending z after the launch.  A tile that
-// lies inside its problem (and whose operands allow 16-byte loads and whose
-// K is a whole number of chunks) runs the unchecked loaders; the others the
-// bounds-checked ones -- the same values either way.
+// grouped_reduce_kernel sums in ascending z after the launch.  In the split
+// modes a tile that lies inside its problem (and whose operands allow 16-byte
+// loads and whose K is a whole number of chunks) runs the unchecked loaders,
+// the others the bounds-checked ones -- the same values either way.
 // ---------------------------------------------------------------------------
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int PIPE_>
 struct TileV {
@@ -838,9 +932,16 @@ __device__ __forceinline__ void grouped_tile(const GroupEntry<typename Traits<MO
   const int t = local - z * per;
   const int tm = t / e.tiles_n;
   const int tn = t - tm * e.tiles_n;
-  const bool inside = e.vec_a && e.vec_b && (tm + 1) * V::BM <= e.M && (tn + 1) * V::BN <= e.N &&
-                      e.K % V::BK == 0;
-  if (inside)
+  // The split modes' kernels have 256 registers per lane and run interior
+  // tiles on the unchecked loaders (BF16x3 512-problem batch 6% faster);
+  // FP32 and FP64 families cap registers at 128, and a second copy of the
+  // tile body made them spill (FP32 3% slower), so they keep one.
+  const bool inside = is_split(MODE) && e.vec_a && e.vec_b && (tm + 1) * V::BM <= e.M &&
+                      (tn + 1) * V::BN <= e.N && e.K % V::BK == 0;
+  if constexpr (!is_split(MODE)) {
+    (void)inside;
+    gemm_tile<MODE, V::BM, V::BN, V::BK, V::WM, V::WN, V::PIPE, false, false, true>(p, tm, tn, z, lds);
+  } else if (inside)
     gemm_tile<MODE, V::BM, V::BN, V::BK, V::WM, V::WN, V::PIPE, false, false, false>(p, tm, tn, z, lds);
   else
     gemm_tile<MODE, V::BM, V::BN, V::BK, V::WM, V::WN, V::PIPE, false, false, true>(p, tm, tn, z, lds);

@@ -164,17 +164,25 @@ def dgemm(A, B, C_out=None, alpha=1.0, beta=0.0, **kw):
     return gemm(A, B, C_out, alpha=alpha, beta=beta, mode="fp64", **kw)
 
 
-def error_stats(Cm: torch.Tensor, R: torch.Tensor, stream=None) -> dict:
-    """FP64 residual statistics of C against a float64 reference R, on the GPU."""
+def error_stats(Cm: torch.Tensor, R: torch.Tensor, stream=None, rel_floor: float = 1e-3) -> dict:
+    """FP64 residual statistics of C against a float64 reference R, on the
+    GPU.  max_rel is max |C-R|/|R| over the entries with |R| >= rel_floor *
+    max|R| (rel_floor 0: every R != 0), so near-zero reference entries do
+    not dominate it; max_rel_all is the unfloored statistic."""
     if R.dtype != torch.float64:
         raise TypeError("R must be float64")
     out = (C.c_double * 5)()
     rows, cols = Cm.shape
-    fn = lib.cosma_error_stats_f32 if Cm.dtype == torch.float32 else lib.cosma_error_stats_f64
-    check(fn(rows, cols, Cm.data_ptr(), _ld(Cm, "C"), R.data_ptr(), _ld(R, "R"), out,
-             _stream_ptr(stream)), "cosma_error_stats")
-    s2, r2, mx, mr, rmax = list(out)
-    return dict(sum_sq_err=s2, sum_sq_ref=r2, max_abs=mx, max_rel=mr, max_ref=rmax)
+    fn = lib.cosma_error_stats_ex_f32 if Cm.dtype == torch.float32 else lib.cosma_error_stats_ex_f64
+    args = (rows, cols, Cm.data_ptr(), _ld(Cm, "C"), R.data_ptr(), _ld(R, "R"))
+    check(fn(*args, 0.0, out, _stream_ptr(stream)), "cosma_error_stats")
+    s2, r2, mx, mr_all, rmax = list(out)
+    mr = mr_all
+    if rel_floor > 0:
+        check(fn(*args, rel_floor * rmax, out, _stream_ptr(stream)), "cosma_error_stats")
+        mr = out[3]
+    return dict(sum_sq_err=s2, sum_sq_ref=r2, max_abs=mx, max_rel=mr, max_rel_all=mr_all, max_ref=rmax,
+                rel_floor=rel_floor)
 
 
 def sumsq(X: torch.Tensor, stream=None) -> float:

@@ -237,6 +237,13 @@ int cosma_error_stats_f32(int rows, int cols, const float *C, int64_t ldc, const
                           int64_t ldr, double *out5, void *stream);
 int cosma_error_stats_f64(int rows, int cols, const double *C, int64_t ldc, const double *R,
                           int64_t ldr, double *out5, void *stream);
+/* The same, with out5[3] the max of |C-R|/|R| over the entries with
+ * |R| >= rel_floor (> 0): a relative error that near-zero reference
+ * entries cannot blow up.  rel_floor = 0 is the plain statistic. */
+int cosma_error_stats_ex_f32(int rows, int cols, const float *C, int64_t ldc, const double *R,
+                             int64_t ldr, double rel_floor, double *out5, void *stream);
+int cosma_error_stats_ex_f64(int rows, int cols, const double *C, int64_t ldc, const double *R,
+                             int64_t ldr, double rel_floor, double *out5, void *stream);
 /* out5[0] = sum x^2 (FP64), out5[2] = max |x|; others 0.  Synchronises. */
 int cosma_sumsq_f32(int rows, int cols, const float *X, int64_t ldx, double *out5, void *stream);
 int cosma_sumsq_f64(int rows, int cols, const double *X, int64_t ldx, double *out5, void *stream);

@@ -398,8 +398,14 @@ def test_residual_stats_on_gpu_match_host(cc, dev):
     st_cpu = oracle.err_stats(Cm, R)
     for key in ("sum_sq_err", "sum_sq_ref"):
         assert np.isclose(st_gpu[key], st_cpu[key], rtol=1e-12)
-    for key in ("max_abs", "max_rel", "max_ref"):
+    for key in ("max_abs", "max_ref"):
         assert st_gpu[key] == st_cpu[key]
+    assert st_gpu["max_rel_all"] == st_cpu["max_rel"]
+    # the floored relative error: entries with |R| >= 1e-3 max|R| only
+    d = np.abs(Cm.astype(np.float64) - R)
+    keep = np.abs(R) >= 1e-3 * np.abs(R).max()
+    assert st_gpu["max_rel"] == (d[keep] / np.abs(R)[keep]).max()
+    assert cc.error_stats(_t(Cm, dev), _t(R, dev), rel_floor=0.0)["max_rel"] == st_cpu["max_rel"]
     # deterministic: bitwise repeatable
     again = cc.error_stats(_t(Cm, dev), _t(R, dev))
     assert again == st_gpu
