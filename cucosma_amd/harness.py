@@ -23,6 +23,8 @@ Configurations:
            fp32, bf16x3, bf16x4 and fp64 (FP64 storage for fp64):
            one hipGraph replay of the library's default grouped plan, of the
            grouped autotuner's plan, and one launch per problem
+  grouped_tiles  uniform batches of 2048^2 outputs, K = 128 .. 2048, on the large
+           tile: per-K-step and fixed per-tile cost of a grouped launch
   autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
@@ -285,6 +287,46 @@ def cfg_grouped(cc, torch, args, emit):
         del As, Bs, Cs
 
 
+def cfg_grouped_tiles(cc, torch, args, emit):
+    """Per-tile cost of a grouped launch: uniform batches of 2048 x 2048
+    outputs with K = 128 .. 2048 on the first family's large tile, timed as
+    graph replays; the slope over K is the cost of a K-step, the intercept the
+    fixed cost of a tile (prologue, epilogue stores)."""
+    modes = args.grouped_modes.split(",") if getattr(args, "grouped_modes", None) else ("bf16x3", "fp32")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for mode in modes:
+        dt = torch.float64 if mode == "fp64" else torch.float32
+        big = cc.group_families(mode)[0][0]
+        rows = []
+        for (cnt, k) in ((64, 128), (64, 256), (32, 1024), (16, 2048)):
+            m = n = 2048
+            As = [torch.rand((m, k), device="cuda", dtype=dt) for _ in range(cnt)]
+            Bs = [torch.rand((k, n), device="cuda", dtype=dt) for _ in range(cnt)]
+            Cs = [torch.empty((m, n), device="cuda", dtype=dt) for _ in range(cnt)]
+            plan = cc.GroupedGemm(As, Bs, Cs, mode=mode, tile_cfgs=[big] * cnt, splits=[1] * cnt)
+            info = plan.info()
+            for _ in range(10):
+                plan.run_graph()
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(args.runs):
+                plan.run_graph()
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / args.runs
+            per_wg = info["total_tiles"] / info["buckets"][0]["blocks"]
+            t = cc.tile_configs(cc._lib.MODES[mode])[big]
+            rows.append((k // t["BK"], ms * 1e3 / per_wg))
+            emit(dict(config="grouped_tiles", mode=mode, tile=f"{t['BM']}x{t['BN']}x{t['BK']}", K=k, problems=cnt,
+                      ms=ms, us_per_tile=ms * 1e3 / per_wg, tflops=cnt * 2.0 * m * n * k / ms / 1e9))
+            plan.close()
+            del As, Bs, Cs
+        x = np.array([r[0] for r in rows], float)
+        y = np.array([r[1] for r in rows], float)
+        slope, icpt = np.polyfit(x, y, 1)
+        emit(dict(config="grouped_tiles_fit", mode=mode, us_per_kstep=float(slope), us_per_tile_fixed=float(icpt)))
+
+
 def cfg_autograd(cc, torch, args, emit):
     g = torch.Generator(device="cpu").manual_seed(4)
     X = (torch.rand(16384, 4096, generator=g) * 2 - 1).cuda().requires_grad_()
@@ -318,7 +360,7 @@ def cfg_autograd(cc, torch, args, emit):
 
 
 CONFIGS = dict(plain=cfg_plain, square=cfg_square, flat=cfg_flat, dgemm=cfg_dgemm,
-               grouped=cfg_grouped, autograd=cfg_autograd, epilogue=cfg_epilogue)
+               grouped=cfg_grouped, grouped_tiles=cfg_grouped_tiles, autograd=cfg_autograd, epilogue=cfg_epilogue)
 
 
 def main(argv=None) -> int:
