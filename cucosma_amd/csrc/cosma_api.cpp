@@ -665,7 +665,7 @@ int64_t padded_area(int mode, int cfg, int m, int n) {
 // tiles read more LDS and global bytes per MFMA than the 256x256 tile's
 // 128x64 waves (padded rate 300 against 217-221 TFLOP/s, 128x128 162).
 constexpr double kVariantWeight[kNumModes][4] = {
-    {1.00, 1.12, 1.08, 1.00},  // FP32: 256x128, 128x256, 128x128 (16 waves)
+    {1.00, 1.08, 1.00, 1.00},  // FP32: 256x128, 128x128 (16 waves)
     {1.00, 1.13, 1.16, 1.27},  // FP64: 128x128, 128x64, 64x128, 64x64
     {1.00, 1.38, 1.35, 1.85},  // BF16X3: 256x256, 256x128, 128x256, 128x128
     {1.00, 1.13, 1.17, 1.59},  // BF16X4: the same tiles
@@ -678,8 +678,8 @@ constexpr double kVariantWeight[kNumModes][4] = {
 //    128x128 tiles, each as one launch;
 //  - otherwise every problem runs in the mode's first tile family (the large
 //    tile and its thin and small variants: BF16 256x256 / 256x128 / 128x256 /
-//    128x128, FP32 256x128 / 128x256 / 128x128, FP64 128x128 / 128x64 /
-//    64x128 / 64x64), each problem on the variant with the least padded
+//    128x128, FP32 256x128 / 128x128, FP64 128x128 / 128x64 / 64x128 /
+//    64x64), each problem on the variant with the least padded
 //    output area times the variant's measured cost per padded flop
 //    (kVariantWeight).  The autotuner refines this per size bucket.
 void group_default_cfgs(int mode, int count, const cosma_group_problem* problems, std::vector<int>& cfg) {

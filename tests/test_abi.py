@@ -292,7 +292,7 @@ def test_every_tile_configuration_in_one_group_family():
         for f in fams:
             assert len({tiles[c]["threads"] for c in f}) == 1, (name, f)
         big = tiles[fams[0][0]]
-        assert len(fams[0]) >= 3
+        assert len(fams[0]) >= 2
         for c in fams[0][1:]:
             assert tiles[c]["BM"] * tiles[c]["BN"] <= big["BM"] * big["BN"]
             assert (tiles[c]["BM"], tiles[c]["BN"]) != (big["BM"], big["BN"])
