@@ -633,8 +633,13 @@ int finish_stats(hipError_t e, const double* dev_out, double* out5, hipStream_t 
 }
 
 // Per-tile cost beyond its K-steps in the grouped schedule (prologue load
-// and C store), in K-steps.
-constexpr int kTileOverheadSteps = 2;
+// and C store), in K-steps.  Swept over 2..10 on the 512-problem batch (one
+// MI355X, two alternating runs each, variant weights applied): 4 was best or
+// within 1.5% of best in every mode.  Uniform batches put the fixed cost of
+// a tile at 2 K-steps (FP32, FP64) to 6 (BF16), but there every workgroup
+// stores C at the same moment; in a mixed batch the stores overlap other
+// workgroups' K loops.
+constexpr int kTileOverheadSteps = 4;
 // Shortest K slice (in K chunks) the default grouped split produces.
 constexpr int kMinSliceSteps = 8;
 
@@ -670,6 +675,22 @@ constexpr double kVariantWeight[kNumModes][4] = {
     {1.00, 1.38, 1.35, 1.85},  // BF16X3: 256x256, 256x128, 128x256, 128x128
     {1.00, 1.13, 1.17, 1.59},  // BF16X4: the same tiles
 };
+
+// Cost of one grouped tile of configuration cfg over ksteps K chunks, in
+// units of padded output elements times K-steps of the first variant: the
+// chunks plus the fixed per-tile cost, times the tile's area, times the
+// variant's measured cost per padded flop (configurations outside the first
+// family, each launched on its own, weigh 1).  The longest-first schedule and
+// the default split both use it; without the weights the thin BF16 tiles
+// were under-priced, and the workgroups that drew many of them finished last
+// (512-problem batch, one box: BF16x3 197 -> 203 TFLOP/s, BF16x4 169 -> 177
+// with the weights and 4 overhead steps).
+double tile_cost(int mode, int cfg, int ksteps) {
+  const TileDims td = kDims[mode][cfg];
+  int fam = -1, v = 0;
+  const double w = family_of(mode, cfg, &fam, &v) && fam == 0 ? kVariantWeight[mode][v] : 1.0;
+  return (double)(ksteps + kTileOverheadSteps) * td.bm * td.bn * w;
+}
 
 // Default tile configuration of every problem of a grouped plan.
 //  - Batches whose flops sit mostly in outputs with both sides below 128
@@ -718,8 +739,9 @@ void group_default_cfgs(int mode, int count, const cosma_group_problem* problems
 }
 
 // Default split-K factor of every problem: a problem is split only when one
-// of its tiles would be a long pole -- its cost (K chunks plus the per-tile
-// overhead, scaled by tile area) more than twice the average load a
+// of its tiles would be a long pole -- its cost (tile_cost: K chunks plus the
+// per-tile overhead, scaled by tile area and variant weight) more than twice
+// the average load a
 // persistent workgroup gets -- into slices of about that average load, at
 // least kMinSliceSteps chunks deep.  On 512 problems with sides in
 // [32, 2048] no problem qualifies; four 256x256x65536 problems split 64 ways
@@ -734,7 +756,7 @@ void group_default_splits(int mode, const std::vector<const cosma_group_problem*
     const cosma_group_problem& q = *probs[i];
     if (q.k <= 0 || q.alpha == 0.0) continue;
     const TileDims td = kDims[mode][cfg[i]];
-    cost[i] = (double)(cdiv(q.k, td.bk) + kTileOverheadSteps) * td.bm * td.bn;
+    cost[i] = tile_cost(mode, cfg[i], cdiv(q.k, td.bk));
     total += cost[i] * cdiv(q.m, td.bm) * (double)cdiv(q.n, td.bn);
   }
   if (slots < 1) slots = 1;
@@ -742,7 +764,7 @@ void group_default_splits(int mode, const std::vector<const cosma_group_problem*
     const cosma_group_problem& q = *probs[i];
     if (cost[i] <= 0.0) continue;
     const TileDims td = kDims[mode][cfg[i]];
-    const double target = std::max(total / slots, (double)kMinSliceSteps * td.bm * td.bn);
+    const double target = std::max(total / slots, tile_cost(mode, cfg[i], kMinSliceSteps) - tile_cost(mode, cfg[i], 0));
     if (cost[i] <= 2.0 * target) continue;
     const int nk = cdiv(q.k, td.bk);
     int s = (int)std::ceil(cost[i] / target);
@@ -781,8 +803,7 @@ namespace {
 // ordered by decreasing tile cost; tiles are numbered problem by problem
 // (K slice, then tile row, then tile column).  The schedule hands them out
 // longest-processing-time first: each tile, in numbering order, goes to the
-// workgroup with the least work so far (tile area times its K chunks plus a
-// fixed per-tile cost for the prologue and the C store), ties to the lowest
+// workgroup with the least work so far (tile_cost), ties to the lowest
 // logical workgroup.  Logical workgroup L runs as workgroup (L % (B/X)) * X
 // + L / (B/X) when B, the grid, is a multiple of X, the device's XCD count:
 // workgroups are dealt to the XCDs round-robin by id, so logical neighbours
@@ -799,15 +820,15 @@ template <typename G>
 int build_bucket(int mode, int fam, std::vector<Member> mem, int blocks_per_cu, cosma_grouped_plan_s::Bucket* out) {
   using E = G;  // accumulator type of the partials (FP32 modes: float, FP64: double)
   const Topo topo = device_topo();
-  auto tile_cost = [&](const Member& m) -> int64_t {
+  auto member_tile_cost = [&](const Member& m) -> int64_t {
     const TileDims td = kDims[mode][m.cfg];
     const int k = m.q->alpha == 0.0 ? 0 : m.q->k;
     const int per = cdiv(k, m.split);
-    const int kslice = cdiv(per, td.bk) * td.bk;
-    return (int64_t)(cdiv(kslice, td.bk) + kTileOverheadSteps) * td.bm * td.bn;
+    return std::llround(tile_cost(mode, m.cfg, cdiv(per, td.bk)));
   };
-  std::stable_sort(mem.begin(), mem.end(),
-                   [&](const Member& a, const Member& b) { return tile_cost(a) > tile_cost(b); });
+  std::stable_sort(mem.begin(), mem.end(), [&](const Member& a, const Member& b) {
+    return member_tile_cost(a) > member_tile_cost(b);
+  });
   std::vector<cosma::GroupEntry<G>> table;
   std::vector<cosma::GroupReduce<G>> red;
   std::vector<int64_t> cost;
@@ -862,7 +883,7 @@ int build_bucket(int mode, int fam, std::vector<Member> mem, int blocks_per_cu, 
       red.push_back(r);
     }
     table.push_back(e);
-    cost.push_back(tile_cost(m));
+    cost.push_back(member_tile_cost(m));
   }
   out->fam = fam;
   out->count = (int)table.size();
