@@ -798,7 +798,7 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
       step(it + 1, la1, lb1, la, lb);
     }
     if (it < nk) step(it, la, lb, la1, lb1);
-  } else if constexpr (PIPE == 4) {
+  } else if constexpr (PIPE == 4 || PIPE == 6) {
     // Ping-pong: waves in the upper half of the workgroup (one per SIMD for
     // 8 waves) copy first -- they write chunk it+1, loaded during the previous
     // iteration, then load chunk it+2 into the freed registers, then run the
@@ -807,7 +807,10 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
     // opposite ends of the iteration, so one wave's VALU/LDS work overlaps
     // the other's MFMAs.  Both write stage it+1 before the barrier and read
     // stage it after the previous one.
-    const bool copy_first = wave >= Sh::WAVES_M * Sh::WAVES_N / 2;
+    // PIPE 6: every wave copies first, so all of the next-but-one chunk is
+    // in flight during the MFMAs (streaming shapes, where memory latency
+    // rather than MFMA issue bounds the loop).
+    const bool copy_first = PIPE == 6 || wave >= Sh::WAVES_M * Sh::WAVES_N / 2;
     if (nk > 0) {
       la.load(0, p.M, kend, tid, p.vec_a);
       lb.load(0, p.N, kend, tid, p.vec_b);
@@ -1025,7 +1028,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// Pre-split BF16 path.  A pass (split_kmajor / split_mnmajor) writes each FP32
+// Pre-split BF16 path.  A pass (split_pair / split_one) writes each FP32
 // operand once as bf16 "planes": row r (an M or N index) of 2*Kp elements,
 // each 32-deep K chunk stored as its 32 hi parts followed by its 32 lo parts
 // (hi = RNE(x), lo = RNE(x - hi), the split the on-the-fly path does), K
@@ -1036,80 +1039,113 @@ __global__ void __launch_bounds__(256)
 // on-the-fly kernel with the same tile, so the results are bitwise equal.
 // ---------------------------------------------------------------------------
 
-// Operand element (r, k) at X[r * ld + k] (k-contiguous rows).  One thread
-// per (row, 8 consecutive k).
-template <int MODE>
-__global__ void __launch_bounds__(256)
-    split_kmajor(const float* __restrict__ X, int64_t ld, int R, int K, int Kp, int vec,
-                 __bf16* __restrict__ out) {
-  const int cpr = Kp / 8;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)R * cpr) return;
+// One operand's split: element (r, k) at X[r * ld + k] when kmajor (k-contiguous
+// rows), else at X[k * ld + r].  blocks: workgroups of 256 threads it takes.
+struct SplitJob {
+  const float* X;
+  int64_t ld;
+  __bf16* out;
+  int R, K, Kp;
+  int kmajor;
+  int vec;     // rows start on 16-byte boundaries and ld % 4 == 0
+  int nrb;     // mn-major: ceil(R / 256) row blocks
+  int blocks;
+};
+
+// k-major operand: one thread per (row, 8 consecutive k); t is the thread's
+// global index.
+__device__ __forceinline__ void split_kmajor_body(const SplitJob& j, int64_t t) {
+  const int cpr = j.Kp / 8;
+  if (t >= (int64_t)j.R * cpr) return;
   const int r = (int)(t / cpr);
   const int k0 = (int)(t - (int64_t)r * cpr) * 8;
-  const float* x = X + (int64_t)r * ld + k0;
+  const float* x = j.X + (int64_t)r * j.ld + k0;
   f32x4 v0, v1;
-  if (vec && k0 + 8 <= K) {
+  if (j.vec && k0 + 8 <= j.K) {
     v0 = *reinterpret_cast<const f32x4*>(x);
     v1 = *reinterpret_cast<const f32x4*>(x + 4);
   } else {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      v0[e] = k0 + e < K ? x[e] : 0.f;
-      v1[e] = k0 + 4 + e < K ? x[4 + e] : 0.f;
+      v0[e] = k0 + e < j.K ? x[e] : 0.f;
+      v1[e] = k0 + 4 + e < j.K ? x[4 + e] : 0.f;
     }
   }
   bf16x4 h0, l0, h1, l1;
   split_bf16x4(v0, h0, l0);
   split_bf16x4(v1, h1, l1);
-  __bf16* o = out + (int64_t)r * 2 * Kp + (k0 / 32) * 64 + (k0 % 32);
+  __bf16* o = j.out + (int64_t)r * 2 * j.Kp + (k0 / 32) * 64 + (k0 % 32);
   *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
   *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
 }
 
-// Operand element (r, k) at X[k * ld + r] (r-contiguous): transposed in
-// registers.  A thread loads 4 consecutive r from each of 8 consecutive k
-// rows (one 16-byte load per row when the rows allow it), so for each of its
-// r it holds the 8 k of one hi and one lo 16-byte chunk of the plane row.
-// Lanes: the 4 k groups of a 32-deep chunk fastest, then 16 groups of 4 r, so
-// each load instruction reads 256 contiguous bytes of 4 k rows and each store
-// instruction fills 64-byte halves of 16 plane lines (hi, then lo).  A block
-// of 4 waves covers 256 r x 32 k.  1-D grid: nrb = ceil(R / 256) row blocks
-// times Kp / 32 k chunks.
-template <int MODE>
-__global__ void __launch_bounds__(256)
-    split_mnmajor(const float* __restrict__ X, int64_t ld, int R, int K, int Kp, int nrb, int vec,
-                  __bf16* __restrict__ out) {
+// mn-major operand (r-contiguous), transposed in registers.  A thread loads
+// 4 consecutive r from each of 8 consecutive k rows (one 16-byte load per row
+// when the rows allow it), so for each of its r it holds the 8 k of one hi
+// and one lo 16-byte chunk of the plane row.  Lanes: the 4 k groups of a
+// 32-deep chunk fastest, then 16 groups of 4 r, so each load instruction
+// reads 256 contiguous bytes of 4 k rows and each store instruction fills
+// 64-byte halves of 16 plane lines (hi, then lo).  Workgroup b covers 256 r
+// x 32 k: row block b % nrb, k chunk b / nrb (row blocks fastest: a
+// chunk-fastest order measured 7-16% slower).
+__device__ __forceinline__ void split_mnmajor_body(const SplitJob& j, int b) {
   const int lane = threadIdx.x & 63;
-  const int r0 = (int)(blockIdx.x % (unsigned)nrb) * 256 + (threadIdx.x >> 6) * 64 + (lane >> 2) * 4;
-  const int chunk = (int)(blockIdx.x / (unsigned)nrb);
+  const int r0 = (b % j.nrb) * 256 + (int)(threadIdx.x >> 6) * 64 + (lane >> 2) * 4;
+  const int chunk = b / j.nrb;
   const int k0 = chunk * 32 + (lane & 3) * 8;
-  if (r0 >= R) return;
-  const bool full = vec && r0 + 4 <= R;
+  if (r0 >= j.R) return;
+  const bool full = j.vec && r0 + 4 <= j.R;
   f32x4 v[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int k = k0 + e;
-    const float* x = X + (int64_t)k * ld + r0;
-    if (full && k < K) {
+    const float* x = j.X + (int64_t)k * j.ld + r0;
+    if (full && k < j.K) {
       v[e] = *reinterpret_cast<const f32x4*>(x);
     } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[e][q] = (k < K && r0 + q < R) ? x[q] : 0.f;
+      for (int q = 0; q < 4; ++q) v[e][q] = (k < j.K && r0 + q < j.R) ? x[q] : 0.f;
     }
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    if (r0 + q >= R) break;
+    if (r0 + q >= j.R) break;
     const f32x4 a{v[0][q], v[1][q], v[2][q], v[3][q]};
-    const f32x4 b{v[4][q], v[5][q], v[6][q], v[7][q]};
+    const f32x4 c{v[4][q], v[5][q], v[6][q], v[7][q]};
     bf16x4 h0, l0, h1, l1;
     split_bf16x4(a, h0, l0);
-    split_bf16x4(b, h1, l1);
-    __bf16* o = out + (int64_t)(r0 + q) * 2 * Kp + chunk * 64 + (lane & 3) * 8;
+    split_bf16x4(c, h1, l1);
+    __bf16* o = j.out + (int64_t)(r0 + q) * 2 * j.Kp + chunk * 64 + (lane & 3) * 8;
     *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
     *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
   }
+}
+
+// Workgroup b of one operand's split.
+__device__ __forceinline__ void split_block(const SplitJob& j, int b) {
+  if (j.kmajor)
+    split_kmajor_body(j, (int64_t)b * 256 + threadIdx.x);
+  else
+    split_mnmajor_body(j, b);
+}
+
+// One operand: j.blocks workgroups.
+template <int MODE>
+__global__ void __launch_bounds__(256) split_one(SplitJob j) {
+  split_block(j, blockIdx.x);
+}
+
+// Both operands in one launch: workgroups [0, a.blocks) split a, the rest b.
+// One launch instead of two saves a kernel boundary and lets the second
+// operand's first workgroups fill the first one's tail (8192^3, A k-major and
+// B mn-major: 0.199 against 0.213 ms for the two launches, same planes).
+template <int MODE>
+__global__ void __launch_bounds__(256) split_pair(SplitJob a, SplitJob b) {
+  const int bid = blockIdx.x;
+  if (bid < a.blocks)
+    split_block(a, bid);
+  else
+    split_block(b, bid - a.blocks);
 }
 
 template <int MODE>

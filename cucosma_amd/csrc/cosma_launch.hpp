@@ -114,23 +114,29 @@ struct Planar {
   }
   static int occupancy(int, int, bool edge) { return edge ? occ<true>() : occ<false>(); }
 
-  // Split operand X into planes: kmajor when element (r, k) is X[r*ld + k].
+  // The split of operand X into planes: kmajor when element (r, k) is
+  // X[r*ld + k].  False when the grid would not fit one launch.
+  static bool split_job(const float* X, int64_t ld, bool kmajor, int R, int K, int Kp, __bf16* out, SplitJob* j) {
+    j->X = X;
+    j->ld = ld;
+    j->out = out;
+    j->R = R;
+    j->K = K;
+    j->Kp = Kp;
+    j->kmajor = kmajor ? 1 : 0;
+    j->vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
+    j->nrb = (R + 255) / 256;
+    const int64_t blocks = kmajor ? ((int64_t)R * (Kp / 8) + 255) / 256 : (int64_t)j->nrb * (Kp / 32);
+    j->blocks = (int)blocks;
+    return blocks <= 0x7fffffff;
+  }
+
   static hipError_t split(const float* X, int64_t ld, bool kmajor, int R, int K, int Kp, __bf16* out,
                           hipStream_t s) {
-    if (kmajor) {
-      const int64_t threads = (int64_t)R * (Kp / 8);
-      const int64_t blocks = (threads + 255) / 256;
-      if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-      const int vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
-      hipLaunchKernelGGL(split_kmajor<MODE>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, R, K, Kp, vec, out);
-    } else {
-      const int nrb = (R + 255) / 256;
-      const int64_t blocks = (int64_t)nrb * (Kp / 32);
-      if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-      const int vec = ((uintptr_t)X % 16 == 0) && (ld % 4 == 0);
-      hipLaunchKernelGGL(split_mnmajor<MODE>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, R, K, Kp, nrb, vec,
-                         out);
-    }
+    SplitJob j;
+    if (!split_job(X, ld, kmajor, R, K, Kp, out, &j)) return hipErrorInvalidValue;
+    if (j.blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_one<MODE>, dim3((unsigned)j.blocks), dim3(256), 0, s, j);
     return hipGetLastError();
   }
 
@@ -140,13 +146,28 @@ struct Planar {
     const __bf16* Ap = static_cast<const __bf16*>(pr.a_planes);
     const __bf16* Bp = static_cast<const __bf16*>(pr.b_planes);
     hipError_t e = hipSuccess;
-    if (!Ap) {
+    if (!Ap && !Bp) {
+      // Both operands: one launch.
+      __bf16* pa = room;
+      __bf16* pb = room + (int64_t)pr.M * 2 * Kp;
+      SplitJob ja, jb;
+      if (!split_job(static_cast<const float*>(pr.A), pr.lda, pr.ta == 0, pr.M, pr.K, Kp, pa, &ja) ||
+          !split_job(static_cast<const float*>(pr.B), pr.ldb, pr.tb == 1, pr.N, pr.K, Kp, pb, &jb) ||
+          (int64_t)ja.blocks + jb.blocks > 0x7fffffff)
+        return hipErrorInvalidValue;
+      if (ja.blocks + jb.blocks > 0) {
+        hipLaunchKernelGGL(split_pair<MODE>, dim3((unsigned)(ja.blocks + jb.blocks)), dim3(256), 0, pr.stream, ja,
+                           jb);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+      }
+      Ap = pa;
+      Bp = pb;
+    } else if (!Ap) {
       e = split(static_cast<const float*>(pr.A), pr.lda, pr.ta == 0, pr.M, pr.K, Kp, room, pr.stream);
       if (e != hipSuccess) return e;
       Ap = room;
-      room += (int64_t)pr.M * 2 * Kp;
-    }
-    if (!Bp) {
+    } else if (!Bp) {
       e = split(static_cast<const float*>(pr.B), pr.ldb, pr.tb == 1, pr.N, pr.K, Kp, room, pr.stream);
       if (e != hipSuccess) return e;
       Bp = room;
