@@ -798,7 +798,7 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
       step(it + 1, la1, lb1, la, lb);
     }
     if (it < nk) step(it, la, lb, la1, lb1);
-  } else if constexpr (PIPE == 4 || PIPE == 6) {
+  } else if constexpr (PIPE == 4) {
     // Ping-pong: waves in the upper half of the workgroup (one per SIMD for
     // 8 waves) copy first -- they write chunk it+1, loaded during the previous
     // iteration, then load chunk it+2 into the freed registers, then run the
@@ -807,10 +807,7 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
     // opposite ends of the iteration, so one wave's VALU/LDS work overlaps
     // the other's MFMAs.  Both write stage it+1 before the barrier and read
     // stage it after the previous one.
-    // PIPE 6: every wave copies first, so all of the next-but-one chunk is
-    // in flight during the MFMAs (streaming shapes, where memory latency
-    // rather than MFMA issue bounds the loop).
-    const bool copy_first = PIPE == 6 || wave >= Sh::WAVES_M * Sh::WAVES_N / 2;
+    const bool copy_first = wave >= Sh::WAVES_M * Sh::WAVES_N / 2;
     if (nk > 0) {
       la.load(0, p.M, kend, tid, p.vec_a);
       lb.load(0, p.N, kend, tid, p.vec_b);
@@ -1138,7 +1135,8 @@ __global__ void __launch_bounds__(256) split_one(SplitJob j) {
 // Both operands in one launch: workgroups [0, a.blocks) split a, the rest b.
 // One launch instead of two saves a kernel boundary and lets the second
 // operand's first workgroups fill the first one's tail (8192^3, A k-major and
-// B mn-major: 0.199 against 0.213 ms for the two launches, same planes).
+// B mn-major, the two orders alternated in one process: 0.176-0.209 against
+// 0.179-0.214 ms for two launches, 2-4% less, same planes).
 template <int MODE>
 __global__ void __launch_bounds__(256) split_pair(SplitJob a, SplitJob b) {
   const int bid = blockIdx.x;
