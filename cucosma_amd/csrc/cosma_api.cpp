@@ -50,7 +50,7 @@ constexpr int kNominalXcds = 8;
 struct TileDims {
   int bm, bn, bk;
 };
-constexpr int kMaxCfg = 12;
+constexpr int kMaxCfg = 13;
 constexpr TileDims kDims[4][kMaxCfg] = {
     // FP32 (cfgs 8, 9: thin and small variants of the 16-wave tile for grouped launches)
     {{128, 128, 32}, {64, 64, 32}, {128, 64, 32}, {256, 128, 32}, {128, 128, 64}, {256, 128, 32},
@@ -58,15 +58,18 @@ constexpr TileDims kDims[4][kMaxCfg] = {
     // FP64 (cfgs 8-10: variants of the 128x128 ping-pong tile)
     {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 16}, {64, 64, 32}, {128, 128, 32},
      {256, 128, 16}, {128, 128, 16}, {128, 64, 16}, {64, 128, 16}, {64, 64, 16}},
-    // BF16X3 (cfg 7, 8: pre-split operands; 9-11: variants of the 256x256 ping-pong tile)
+    // BF16X3 (cfg 7, 8: pre-split operands; 9-11: variants of the 256x256 ping-pong tile; 12: the
+    // 256x256 tile with loads spread among the MFMAs)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32}},
+     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32},
+     {256, 256, 32}},
     // BF16X4 (same tiles as BF16X3)
     {{128, 128, 32}, {64, 64, 32}, {256, 128, 32}, {256, 256, 32}, {256, 128, 32}, {128, 128, 64},
-     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32}},
+     {256, 256, 32}, {256, 256, 32}, {256, 256, 32}, {256, 128, 32}, {128, 256, 32}, {128, 128, 32},
+     {256, 256, 32}},
 };
 constexpr int kNumModes = 4;
-constexpr int kNumCfg[kNumModes] = {10, 11, 12, 12};
+constexpr int kNumCfg[kNumModes] = {10, 11, 13, 13};
 
 constexpr int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -144,12 +147,19 @@ constexpr bool split_mode(int mode) { return mode == 2 || mode == 3; }
 // call against 352 with 8 waves, 3072^3 and 6144^3 +12-16%).
 constexpr bool is_planar(int mode, int cfg) { return split_mode(mode) && (cfg == 7 || cfg == 8); }
 constexpr int planar_cfg(int mode) { (void)mode; return 8; }
-constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : 6); }
+// BF16: the 256x256 tile whose next-chunk loads are spread among the MFMAs
+// (cfg 12; same products in the same order as the ping-pong cfg 6, so the
+// same bits).  Cache-flushed, the two orders alternated on one box: BF16x3
+// 3-12% faster than cfg 6 on nine shapes (8192^3 on the fly 2.298 against
+// 2.394 ms, 256x256x1048576 0.518 against 0.542, 1024^2x32768 0.194 against
+// 0.221), BF16x4 0-3% on seven of nine, 256x256x1048576 12% (0.544 against
+// 0.617), 2048^2x16384 1.7% slower.
+constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : 12); }
 constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
 // Deep-K tile: the large tile when the output fills it, else a 128-row tile
 // with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
 constexpr int deep_cfg(int mode, int m, int n) {
-  const int big = mode == 0 ? 7 : (mode == 1 ? 2 : 6);
+  const int big = mode == 0 ? 7 : (mode == 1 ? 2 : 12);
   if (m >= kDims[mode][big].bm && n >= kDims[mode][big].bn) return big;
   return split_mode(mode) ? 5 : 2;
 }

@@ -34,11 +34,13 @@ using B3C8 = Planar<kBF16x3, 256, 256, 64, 64>;
 using B3C9 = Tile<kBF16x3, 256, 128, 32, 64, 64, 4>;
 using B3C10 = Tile<kBF16x3, 128, 256, 32, 64, 64, 4>;
 using B3C11 = Tile<kBF16x3, 128, 128, 32, 64, 32, 4>;
+using B3C12 = Tile<kBF16x3, 256, 256, 32, 128, 64, 7>;
 
-using B3Tiles = TileTable<B3C0, B3C1, B3C2, B3C3, B3C4, B3C5, B3C6, B3C7, B3C8, B3C9, B3C10, B3C11>;
+using B3Tiles = TileTable<B3C0, B3C1, B3C2, B3C3, B3C4, B3C5, B3C6, B3C7, B3C8, B3C9, B3C10, B3C11, B3C12>;
 using B3Fams = FamilyTable<Family<kBF16x3, B3Tiles, 6, 9, 10, 11>, Family<kBF16x3, B3Tiles, 0>,
                            Family<kBF16x3, B3Tiles, 1>, Family<kBF16x3, B3Tiles, 2>, Family<kBF16x3, B3Tiles, 3>,
-                           Family<kBF16x3, B3Tiles, 4>, Family<kBF16x3, B3Tiles, 5>>;
+                           Family<kBF16x3, B3Tiles, 4>, Family<kBF16x3, B3Tiles, 5>,
+                           Family<kBF16x3, B3Tiles, 12>>;
 
 // Host-side dispatch of this mode (a function-local object: a namespace-scope
 // constant would also be emitted for the device).

@@ -26,11 +26,13 @@ using B4C8 = Planar<kBF16x4, 256, 256, 64, 64>;
 using B4C9 = Tile<kBF16x4, 256, 128, 32, 64, 64, 4>;
 using B4C10 = Tile<kBF16x4, 128, 256, 32, 64, 64, 4>;
 using B4C11 = Tile<kBF16x4, 128, 128, 32, 64, 32, 4>;
+using B4C12 = Tile<kBF16x4, 256, 256, 32, 128, 64, 7>;
 
-using B4Tiles = TileTable<B4C0, B4C1, B4C2, B4C3, B4C4, B4C5, B4C6, B4C7, B4C8, B4C9, B4C10, B4C11>;
+using B4Tiles = TileTable<B4C0, B4C1, B4C2, B4C3, B4C4, B4C5, B4C6, B4C7, B4C8, B4C9, B4C10, B4C11, B4C12>;
 using B4Fams = FamilyTable<Family<kBF16x4, B4Tiles, 6, 9, 10, 11>, Family<kBF16x4, B4Tiles, 0>,
                            Family<kBF16x4, B4Tiles, 1>, Family<kBF16x4, B4Tiles, 2>, Family<kBF16x4, B4Tiles, 3>,
-                           Family<kBF16x4, B4Tiles, 4>, Family<kBF16x4, B4Tiles, 5>>;
+                           Family<kBF16x4, B4Tiles, 4>, Family<kBF16x4, B4Tiles, 5>,
+                           Family<kBF16x4, B4Tiles, 12>>;
 
 // Host-side dispatch of this mode (a function-local object: a namespace-scope
 // constant would also be emitted for the device).

@@ -313,6 +313,24 @@ struct TileLoader {
     }
   }
 
+  // The same loads one instruction at a time: load_one(j, ...) for j in
+  // [0, NREG) issues the j-th of load()'s vector loads, so a caller can spread
+  // them among its MFMAs.
+  __device__ __forceinline__ void load_one(int j, int kofs, int rmax, int kend, int tid, bool vec_ok) {
+    const int i = KMAJ ? j : j / GVEC;
+    if (!active(tid + i * NT)) return;
+    const int rr = r0_ + i * UNIT_STEP;
+    const int gk = k0_ + kofs;
+    if constexpr (KMAJ) {
+      gptr<const G> p = p0 + (int64_t)(i * UNIT_STEP) * ld_ + kofs;
+      v[i] = load_vec(p, rr < rmax, kend - gk, vec_ok);
+    } else {
+      const int u = j - i * GVEC;
+      gptr<const G> p = p0 + (int64_t)(kofs + u) * ld_ + i * UNIT_STEP;
+      v[j] = load_vec(p, gk + u < kend, rmax - rr, vec_ok);
+    }
+  }
+
   // Writes the registers to LDS plane(s) sh (and sl for the bf16 lo part).
   __device__ __forceinline__ void store(S* __restrict__ sh, S* __restrict__ sl, int tid) const {
     if constexpr (KMAJ) {
@@ -467,10 +485,11 @@ __device__ __forceinline__ void split_products_one(f32x4& acc, const bf16x8& ah,
 //      after, each calling this with the PIPE 0 order.
 // Which is faster depends on the tile (register headroom, MFMA latency of the
 // type); each tile configuration fixes its choice from measurement.
-template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, typename Mid>
+template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, typename Mid, typename Row = void (*)(int)>
 __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __restrict__ st,
                                               typename Traits<MODE>::Acc (&acc)[WM / 16][WN / 16],
-                                              int wm0, int wn0, int lane, Mid&& mid) {
+                                              int wm0, int wn0, int lane, Mid&& mid,
+                                              Row&& after_row = [](int) {}) {
   using Sh = TileShape<MODE, BM, BN, BK, WM, WN>;
   using T = Traits<MODE>;
   using Frag = typename T::Frag;
@@ -530,6 +549,7 @@ __device__ __forceinline__ void compute_stage(const typename Traits<MODE>::S* __
             split_products_one<MODE>(acc[i][j], ah, al, bh[j], bl[j]);
             __builtin_amdgcn_sched_group_barrier(0x008, MODE == kBF16x4 ? 4 : 3, 0);
           }
+          if (q == NQ - 1) after_row(i);
         }
       } else {
 #pragma unroll
@@ -835,6 +855,40 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
         compute_stage<MODE, BM, BN, BK, WM, WN, 0>(stage_ptr(cur), acc, wm0, wn0, lane,
                                                    [&]() { store_to(la, lb, nxt); });
       }
+      __syncthreads();
+    }
+  } else if constexpr (PIPE == 7) {
+    // Every wave copies first: it writes chunk it+1 (loaded during the
+    // previous iteration) to the free stage, then runs the MFMAs of chunk it
+    // with the loads of chunk it+2 spread among them, one group after each
+    // sub-tile row.  A wave's loads are capped by the memory pipeline's
+    // queue: issued together before the MFMAs, the later ones wait for the
+    // earlier ones to drain and hold back the MFMAs behind them.
+    constexpr int NLA = decltype(la)::NREG, NLB = decltype(lb)::NREG, NL = NLA + NLB;
+    if (nk > 0) {
+      la.load(0, p.M, kend, tid, p.vec_a);
+      lb.load(0, p.N, kend, tid, p.vec_b);
+      store_to(la, lb, stage_ptr(0));
+      const int k1 = (nk > 1 ? 1 : 0) * BK;
+      la.load(k1, p.M, kend, tid, p.vec_a);
+      lb.load(k1, p.N, kend, tid, p.vec_b);
+      __syncthreads();
+    }
+    for (int it = 0; it < nk; ++it) {
+      const int cur = it & 1;
+      store_to(la, lb, stage_ptr(cur ^ 1));
+      const int kofs = min(it + 2, nk - 1) * BK;
+      compute_stage<MODE, BM, BN, BK, WM, WN, 0>(stage_ptr(cur), acc, wm0, wn0, lane, []() {}, [&](int i) {
+#pragma unroll
+        for (int j = i * NL / Sh::TM; j < (i + 1) * NL / Sh::TM; ++j) {
+          if (j < NLA)
+            la.load_one(j, kofs, p.M, kend, tid, p.vec_a);
+          else
+            lb.load_one(j - NLA, kofs, p.N, kend, tid, p.vec_b);
+        }
+        constexpr int kPerRow = NL / Sh::TM > 0 ? NL / Sh::TM : 1;
+        __builtin_amdgcn_sched_group_barrier(0x020, kPerRow, 0);  // these loads, after this row's MFMAs
+      });
       __syncthreads();
     }
   } else {

@@ -94,8 +94,8 @@ def test_heuristic_round_rules():
     for mode in (2, 3):
         assert h(mode, 4608, 4608, 4096) == (8, 1)
         assert h(mode, 4096, 12288, 4096) == (8, 1)
-        assert h(mode, 4096, 4096, 4096) == (6, 1)
-        assert h(mode, 5632, 5632, 2048) == (6, 1)
+        assert h(mode, 4096, 4096, 4096) == (12, 1)
+        assert h(mode, 5632, 5632, 2048) == (12, 1)
         assert h(mode, 8192, 8192, 8192) == (8, 1)
 
 

@@ -274,8 +274,10 @@ def test_fused_epilogue(cc, dev, mode, dt, act):
         cc.gemm(Ad, Bd, Cd, mode=mode, epilogue="tanh")
 
 
-# pre-split configurations and the on-the-fly configuration with the same 256x256 tile
-PRESPLIT = {"bf16x3": ((7, 8), 6), "bf16x4": ((7, 8), 6)}
+# pre-split configurations, and the on-the-fly configurations with the same
+# 256x256 tile (the default one first: loads spread among the MFMAs; then the
+# ping-pong one of grouped launches)
+PRESPLIT = {"bf16x3": ((7, 8), (12, 6)), "bf16x4": ((7, 8), (12, 6))}
 
 
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
@@ -285,11 +287,12 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
     loads) computes the same products in the same order as the on-the-fly
     kernel with the same 256x256 tile: bitwise equal results, for aligned and
     ragged shapes, with split-K, alpha and beta."""
-    planars, same = PRESPLIT[mode]
+    planars, fly = PRESPLIT[mode]
+    same = fly[0]
     t = cc.tile_configs(cc._lib.MODES[mode])
     assert [c["cfg"] for c in t if c["pipe"] == 5] == list(planars)
-    for planar in planars:
-        assert (t[planar]["BM"], t[planar]["BN"]) == (t[same]["BM"], t[same]["BN"])
+    for cfg in planars + fly[1:]:
+        assert (t[cfg]["BM"], t[cfg]["BN"]) == (t[same]["BM"], t[same]["BN"])
     rng = np.random.default_rng(400)
     for (M, N, K, split) in ((512, 512, 256, 1), (300, 200, 100, 1), (256, 512, 1000, 3), (33, 517, 77, 2)):
         A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
@@ -297,7 +300,7 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
         C0 = rng.uniform(-1, 1, (M, N)).astype(np.float32)
         Ad, Bd = _t(A.T if ta else A, dev), _t(B.T if tb else B, dev)
         outs = []
-        for cfg in (same,) + planars:
+        for cfg in fly + planars:
             Cd = _t(C0, dev)
             cc.gemm(Ad, Bd, Cd, alpha=1.25, beta=0.5, mode=mode, trans_a=ta, trans_b=tb, tile_cfg=cfg,
                     split_k=split)
@@ -322,7 +325,8 @@ def test_heuristic_presplit_for_underfilled_tiles(cc, dev, mode):
     """Outputs whose 256x256 tiles fill between half the CUs and all of them
     take the pre-split configuration by default (library workspace for the
     planes); the result is bitwise the on-the-fly 256x256 tile's."""
-    planars, same = PRESPLIT[mode]
+    planars, fly = PRESPLIT[mode]
+    same = fly[0]
     M, N, K = 3072, 3072, 256
     cfg, split = cc.heuristic(cc._lib.MODES[mode], M, N, K)
     assert cfg in planars and split == 1
