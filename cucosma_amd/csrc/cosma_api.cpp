@@ -98,12 +98,13 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    with partly filled or few rounds, 6-8% faster in both split modes;
 //    4096^3, 2048x8192x2048 and 4096x8192x4096 tie or lose, and keep cfg 6;
 //    5632^2x2048 lost 1% and is just outside).  5504 >= 2^24 / 3072, so every
-//    output with M*N/(M+N) >= 3072 also qualifies.  The pre-split tile is
-//    also taken when its 256x256 tiles number between half the CUs and all
-//    of them (one workgroup per CU, some CUs idle): there the 16-wave kernel's
-//    faster K loop outweighs the split pass (BF16x3 +2% to +17% over nine
-//    shapes from 2048x4096x4096 to 3584^2x4096, row-major and A transposed;
-//    BF16x4 -3% to +3% row-major, +5% to +13% with A transposed);
+//    output with M*N/(M+N) >= 3072 also qualifies (BF16x4: 6144).  The
+//    pre-split tile is also taken when its 256x256 tiles number between half
+//    the CUs and all of them (one workgroup per CU, some CUs idle) and K is at
+//    least 512: there the 16-wave kernel's faster K loop outweighs the split
+//    pass (against the spread-load tile: 2048x4096x4096 and 3072^2x4096
+//    BF16x3 0.5-1% and BF16x4 6-8% faster, 3584^2x4096 BF16x3 3% slower and
+//    BF16x4 3% faster; at K = 256 the spread-load tile is 11-18% faster);
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    in 16 waves of 64x32 with the next stage held in registers (four waves
 //    per SIMD; 3-5% above the 8-wave version, which itself matched the
@@ -170,12 +171,11 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   return split < 1 ? 1 : split;
 }
 // ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A
-// in their loaders; the pre-split pass does it once, so with a transposed A the
-// pre-split tile wins even at one round of workgroups (BF16x3 4096x4096x16384,
-// the weight gradient of a 16384x4096x4096 layer: 1.185 ms against 1.353 for
-// cfg 6; BF16x4 1.562 against 1.606; 4096^2x1024 0.91x, 6144^2x1024 0.77x,
-// 2048x8192x2048 0.89x the cfg 6 time).  Not for K < 512, where the split
-// pass and its launches outweigh it (4096^2x256: 1.15x).
+// in their loaders; the pre-split pass does it once, so with a transposed A and
+// a deep K the pre-split tile is taken even at one round of workgroups (the
+// weight gradient of a 16384x4096x4096 layer, 4096x4096x16384: against the
+// spread-load tile, BF16x4 1.7% faster, BF16x3 1.8% slower; at K = 1024 the
+// spread-load tile is 9% / 3% faster, so the rule starts at K = 4096).
 constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
   // An empty output launches nothing; a non-positive CU count is not a device.
   if (m <= 0 || n <= 0 || cus <= 0) return Choice{medium_cfg(mode), 1};
@@ -191,7 +191,18 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
     if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= cus / 2 && !thin) {
       // Large outputs, and outputs whose tiles leave CUs idle: split the
       // operands once and copy planes into LDS.
-      if (split == 1 && (lt < cus || (ta && k >= 512) || (int64_t)cdiv(lt, cus) * 5504 >= (int64_t)m + n))
+      // Against the spread-load on-the-fly tile (cfg 12; cache flushed, the two
+      // orders alternated on one box): nothing pre-splits below K = 512, where
+      // the extra launch outweighs the faster loop (3072^2x256: cfg 12 18% / 11%
+      // faster in BF16x3 / BF16x4; 3072^2x4096: the pre-split tile 0.5% / 6%,
+      // one partly filled round); A stored transposed from K
+      // = 4096 on (4096^2x1024: cfg 12 9% / 3% faster); and the rounds rule's
+      // constant is 5504 for BF16x3 and 6144 for BF16x4, whose pre-split kernel
+      // gains more per round (5632^2x2048 and 4096x8192x4096: BF16x4 pre-split
+      // 3-4% faster, BF16x3 within 1.6% either way).
+      const int64_t per_round = mode == 3 ? 6144 : 5504;
+      if (split == 1 && k >= 512 &&
+          (lt < cus || (ta && k >= 4096) || (int64_t)cdiv(lt, cus) * per_round >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, split};
     }

@@ -95,18 +95,24 @@ def test_heuristic_round_rules():
         assert h(mode, 4608, 4608, 4096) == (8, 1)
         assert h(mode, 4096, 12288, 4096) == (8, 1)
         assert h(mode, 4096, 4096, 4096) == (12, 1)
-        assert h(mode, 5632, 5632, 2048) == (12, 1)
+        # BF16x4's rounds constant is 6144: two rounds reach M + N = 11264.
+        assert h(mode, 5632, 5632, 2048) == ((12, 1) if mode == 2 else (8, 1))
+        assert h(mode, 4096, 8192, 4096) == ((12, 1) if mode == 2 else (8, 1))
+        # One partly filled round pre-splits only from K = 512 on.
+        assert h(mode, 3072, 3072, 1024) == (8, 1)
+        assert h(mode, 3072, 3072, 256) == (12, 1)
         assert h(mode, 8192, 8192, 8192) == (8, 1)
 
 
 def test_transposed_a_takes_the_presplit_tile():
-    """With A stored transposed (K >= 512) the split modes take the pre-split
+    """With A stored transposed (K >= 4096) the split modes take the pre-split
     tile, so the call needs workspace for the planes: (M + N) rows of 2 * K
     bf16.  Row-major A at one round keeps the on-the-fly tile (no workspace)."""
     import cucosma_amd
 
     for mode in (2, 3):
-        for ta, k, want in ((1, 16384, (4096 + 4096) * 2 * 16384 * 2), (0, 16384, 0), (1, 256, 0)):
+        for ta, k, want in ((1, 16384, (4096 + 4096) * 2 * 16384 * 2), (0, 16384, 0), (1, 1024, 0),
+                            (1, 256, 0)):
             d = _desc(mode=mode, m=4096, n=4096, k=k, trans_a=ta, lda=4096 if ta else k, ldb=4096, ldc=4096)
             out = C.c_size_t()
             assert cucosma_amd.lib.cosma_workspace_query(C.byref(d), C.byref(out)) == 0

@@ -323,11 +323,13 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
 def test_heuristic_presplit_for_underfilled_tiles(cc, dev, mode):
     """Outputs whose 256x256 tiles fill between half the CUs and all of them
-    take the pre-split configuration by default (library workspace for the
-    planes); the result is bitwise the on-the-fly 256x256 tile's."""
+    take the pre-split configuration by default from K = 512 on (library
+    workspace for the planes); the result is bitwise the on-the-fly 256x256
+    tile's."""
     planars, fly = PRESPLIT[mode]
     same = fly[0]
-    M, N, K = 3072, 3072, 256
+    M, N, K = 3072, 3072, 1024
+    assert cc.heuristic(cc._lib.MODES[mode], M, N, 256)[0] == same
     cfg, split = cc.heuristic(cc._lib.MODES[mode], M, N, K)
     assert cfg in planars and split == 1
     assert cc.heuristic(cc._lib.MODES[mode], 4096, 4096, 4096)[0] == same
