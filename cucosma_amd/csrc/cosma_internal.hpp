@@ -43,7 +43,8 @@ struct TileInfo {
   // Next-stage copy placement: 0 after the MFMAs, 1 after half of them, 2
   // after three quarters, 3 register double-buffered, 4 ping-pong (half the
   // waves copy before their MFMAs, half after), 5 pre-split operands copied
-  // with direct-to-LDS loads (no grouped launch).
+  // with direct-to-LDS loads (no grouped launch), 7 every wave copies first
+  // and issues its next loads one by one among its MFMAs.
   int pipe;
 };
 

@@ -482,7 +482,10 @@ __device__ __forceinline__ void split_products_one(f32x4& acc, const bf16x8& ah,
 //      double-buffers the loader registers so mid()'s data arrived an
 //      iteration earlier (gemm_tile);
 //   4  ping-pong (gemm_tile): half the waves copy before their MFMAs, half
-//      after, each calling this with the PIPE 0 order.
+//      after, each calling this with the PIPE 0 order;
+//   7  spread loads (gemm_tile): every wave copies first, then calls this
+//      with the PIPE 0 order and an after_row hook that issues the next
+//      loads one scheduling group per sub-tile row.
 // Which is faster depends on the tile (register headroom, MFMA latency of the
 // type); each tile configuration fixes its choice from measurement.
 template <int MODE, int BM, int BN, int BK, int WM, int WN, int PIPE, typename Mid, typename Row = void (*)(int)>

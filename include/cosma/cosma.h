@@ -208,7 +208,9 @@ int cosma_sgemm_bf16x3_mnk(int m, int n, int k, float alpha, const float *A, int
  * WN, threads per workgroup, LDS bytes, pipeline schedule (0: next-stage copy
  * after the MFMAs, 1: after half of them, 2: after three quarters, 3:
  * register double-buffered, 4: ping-pong -- half the waves copy before their
- * MFMAs, half after).  Replaces the tile
+ * MFMAs, half after, 5: pre-split operands copied with direct-to-LDS loads,
+ * 7: every wave copies first and spreads its next loads among its MFMAs).
+ * Replaces the tile
  * macros of config.h (THREADBLOCK_TILE_*, WARP_TILE_*, LOAD_K). */
 int cosma_num_tile_configs(int mode);
 int cosma_tile_config(int mode, int cfg, int *out8);
