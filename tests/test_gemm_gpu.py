@@ -343,6 +343,40 @@ def test_heuristic_presplit_for_underfilled_tiles(cc, dev, mode):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), ta
 
 
+@pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
+def test_spread_load_tile_equals_ping_pong(cc, dev, mode):
+    """The spread-load 256x256 tile (every wave copies first and issues its
+    next loads among its MFMAs) computes the ping-pong tile's products in the
+    same order: bitwise equal results over random shapes (edges, K not a
+    multiple of 32, unaligned leading dimensions), layouts, split-K factors,
+    alpha/beta and fused epilogues."""
+    planars, fly = PRESPLIT[mode]
+    spread, pingpong = fly
+    t = cc.tile_configs(cc._lib.MODES[mode])
+    assert t[spread]["pipe"] == 7 and t[pingpong]["pipe"] == 4
+    rng = np.random.default_rng(404)
+    for case in range(12):
+        M, N = (int(x) for x in rng.integers(1, 700, 2))
+        K = int(rng.integers(1, 3000))
+        ta, tb = LAYOUTS[case % 3]
+        split = int(rng.integers(1, 6))
+        pad = int(rng.integers(0, 3))  # extra columns: unaligned rows for pad 1, 2
+        alpha, beta = float(rng.uniform(-2, 2)), (0.0 if case % 4 == 0 else float(rng.uniform(-1, 1)))
+        act = (None, "relu", "sigmoid")[case % 3]
+        A = rng.uniform(-1, 1, (K, M + pad) if ta else (M, K + pad)).astype(np.float32)
+        B = rng.uniform(-1, 1, (N, K + pad) if tb else (K, N + pad)).astype(np.float32)
+        C0 = rng.uniform(-1, 1, (M, N)).astype(np.float32)
+        Ad = _t(A, dev)[:, :M] if ta else _t(A, dev)[:, :K]
+        Bd = _t(B, dev)[:, :K] if tb else _t(B, dev)[:, :N]
+        outs = []
+        for cfg in (spread, pingpong):
+            Cd = _t(C0, dev)
+            cc.gemm(Ad, Bd, Cd, alpha=alpha, beta=beta, mode=mode, trans_a=ta, trans_b=tb, tile_cfg=cfg,
+                    split_k=split, epilogue=act)
+            outs.append(Cd.cpu().numpy())
+        assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), (M, N, K, ta, tb, split, pad)
+
+
 @pytest.mark.parametrize("ta,tb", LAYOUTS)
 def test_presplit_caller_planes(cc, dev, ta, tb):
     """Operands split once by split_operand and passed as planes give the
