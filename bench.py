@@ -210,7 +210,7 @@ def main() -> int:
     shape_b = (f"{tile_b['BM']}x{tile_b['BN']}x{tile_b['BK']}, "
                f"{tile_b['threads'] // 64} waves of {tile_b['WM']}x{tile_b['WN']}")
     if presplit:
-        # Each step splits A and B into bf16 planes (two HBM-bound passes) and
+        # Each step splits A and B into bf16 planes (one HBM-bound launch) and
         # runs planar_kernel on them.  The roofline is the dominant kernel's:
         # split once, then time planar_kernel launches alone with HIP events.
         kname = f"planar_kernel<BF16x3, {shape_b}>"
@@ -222,7 +222,7 @@ def main() -> int:
 
         _, ev_k, _ = time_launches(run_kernel, stream, args.steps, args.warmup, torch)
         log(f"planar_kernel alone: {ev_k:.3f} ms/launch ({flops / ev_k / 1e9:.1f} TFLOP/s); "
-            f"split passes {ev_b - ev_k:.3f} ms per step")
+            f"split pass {ev_b - ev_k:.3f} ms per step")
         del ap, bp
     else:
         kname = f"gemm_kernel<BF16x3, {shape_b}, pipe {tile_b['pipe']}>"
