@@ -26,6 +26,8 @@ Configurations:
   grouped_tiles  uniform batches of 2048^2 outputs, K = 128 .. 2048, on the large
            tile: per-K-step and fixed per-tile cost of a grouped launch
   autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
+  schedules the two on-the-fly 256x256 BF16 schedules (ping-pong cfg 6, spread
+           loads cfg 12) on five shapes, orders alternated round by round
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
 """
@@ -359,8 +361,40 @@ def cfg_autograd(cc, torch, args, emit):
                   pct_peak_step=100 * 3 * flops / (fwd + bwd) / 1e9 / PEAK[mode]))
 
 
+def cfg_schedules(cc, torch, args, emit):
+    """The two on-the-fly 256x256 BF16 schedules side by side: ping-pong
+    copies (cfg 6, the grouped launches' large tile) against next-chunk loads
+    spread among the MFMAs (cfg 12, the single-launch large tile).  Same
+    products in the same order (bitwise equal); the orders alternate from
+    round to round and the first round is dropped, since a configuration timed
+    later in a process tends to run faster."""
+    shapes = [(256, 256, 1 << 20, 256), (8192, 8192, 8192, 1), (4096, 4096, 4096, 1),
+              (1024, 1024, 32768, 16), (2048, 2048, 16384, 4)]
+    rounds = 6
+    for mode in ("bf16x3", "bf16x4"):
+        for (M, N, K, split) in shapes:
+            g = torch.Generator(device="cuda").manual_seed(1)
+            A = torch.rand((M, K), device="cuda", generator=g) * 2 - 1
+            B = torch.rand((K, N), device="cuda", generator=g) * 2 - 1
+            Cm = torch.empty((M, N), device="cuda")
+            res = {6: [], 12: []}
+            for r in range(rounds):
+                for cfg in ((6, 12) if r % 2 == 0 else (12, 6)):
+                    ms = bench_gemm(cc, mode, A, B, Cm, runs=max(3, args.runs // 10), warmup=1,
+                                    flush=not args.no_flush, split_k=split, tile_cfg=cfg)
+                    if r > 0:
+                        res[cfg] += list(ms)
+            m6, m12 = float(np.median(res[6])), float(np.median(res[12]))
+            flops = 2.0 * M * N * K
+            emit(dict(config="schedules", mode=mode, M=M, N=N, K=K, split_k=split,
+                      ms_pingpong_cfg6=m6, ms_spread_cfg12=m12, ratio_cfg12_over_cfg6=m12 / m6,
+                      tflops_cfg6=flops / m6 / 1e9, tflops_cfg12=flops / m12 / 1e9))
+            del A, B, Cm
+
+
 CONFIGS = dict(plain=cfg_plain, square=cfg_square, flat=cfg_flat, dgemm=cfg_dgemm,
-               grouped=cfg_grouped, grouped_tiles=cfg_grouped_tiles, autograd=cfg_autograd, epilogue=cfg_epilogue)
+               grouped=cfg_grouped, grouped_tiles=cfg_grouped_tiles, autograd=cfg_autograd, epilogue=cfg_epilogue,
+               schedules=cfg_schedules)
 
 
 def main(argv=None) -> int:
