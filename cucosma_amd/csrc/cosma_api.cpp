@@ -1368,7 +1368,8 @@ size_t cosma_planes_bytes(int rows, int k) {
 
 int cosma_split_operand(int mode, int kmajor, int rows, int k, const float* X, int64_t ld, void* planes,
                         void* stream) {
-  if (!split_mode(mode) || rows < 0 || k < 0) return COSMA_STATUS_INVALID_VALUE;
+  // k rounded up to a whole 32-deep chunk must fit an int.
+  if (!split_mode(mode) || rows < 0 || k < 0 || k > 0x7fffffe0) return COSMA_STATUS_INVALID_VALUE;
   if (rows == 0 || k == 0) return 0;
   if (!X || !planes) return COSMA_STATUS_INVALID_VALUE;
   const int64_t cols = kmajor ? k : rows;

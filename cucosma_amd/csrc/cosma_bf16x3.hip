@@ -52,6 +52,7 @@ const ModeOps* ops_bf16x3() {
 hipError_t run_split_operand(int mode, bool kmajor, int rows, int K, const float* X, int64_t ld, void* planes,
                              hipStream_t s) {
   (void)mode;  // the split is the same for three and four products
+  if (K > 0x7fffffe0) return hipErrorInvalidValue;  // Kp, K rounded up to 32, must fit an int
   const int Kp = (int)(((int64_t)K + 31) / 32 * 32);
   return B3C7::split(X, ld, kmajor, rows, K, Kp, static_cast<__bf16*>(planes), s);
 }

@@ -1128,7 +1128,7 @@ __device__ __forceinline__ void split_kmajor_body(const SplitJob& j, int64_t t) 
   bf16x4 h0, l0, h1, l1;
   split_bf16x4(v0, h0, l0);
   split_bf16x4(v1, h1, l1);
-  __bf16* o = j.out + (int64_t)r * 2 * j.Kp + (k0 / 32) * 64 + (k0 % 32);
+  __bf16* o = j.out + (int64_t)r * 2 * j.Kp + (int64_t)(k0 / 32) * 64 + (k0 % 32);
   *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
   *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
 }
@@ -1169,7 +1169,7 @@ __device__ __forceinline__ void split_mnmajor_body(const SplitJob& j, int b) {
     bf16x4 h0, l0, h1, l1;
     split_bf16x4(a, h0, l0);
     split_bf16x4(c, h1, l1);
-    __bf16* o = j.out + (int64_t)(r0 + q) * 2 * j.Kp + chunk * 64 + (lane & 3) * 8;
+    __bf16* o = j.out + (int64_t)(r0 + q) * 2 * j.Kp + (int64_t)chunk * 64 + (lane & 3) * 8;
     *reinterpret_cast<bf16x8*>(o) = bf16x8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
     *reinterpret_cast<bf16x8*>(o + 32) = bf16x8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
   }

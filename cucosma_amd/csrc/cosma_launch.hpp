@@ -141,6 +141,7 @@ struct Planar {
   }
 
   static hipError_t run(const Problem& pr) {
+    if (pr.K > 0x7fffffe0) return hipErrorInvalidValue;  // Kp, K rounded up to 32, must fit an int
     const int Kp = (int)(((int64_t)pr.K + 31) / 32 * 32);
     __bf16* room = static_cast<__bf16*>(pr.planes);
     const __bf16* Ap = static_cast<const __bf16*>(pr.a_planes);

@@ -184,6 +184,10 @@ def test_empty_problems_are_noops_on_host():
     m, n, k = C.c_int(), C.c_int(), C.c_int()
     lib.cosma_fixed_shape(C.byref(m), C.byref(n), C.byref(k))
     assert (m.value, n.value, k.value) == (1024, 1024, 1024)
+    # split_operand: K rounded up to a 32-deep chunk must fit an int; empty is a no-op
+    assert lib.cosma_split_operand(2, 1, 4, 0x7FFFFFF0, 0x1000, 0x7FFFFFF0, 0x2000, None) == -1
+    assert lib.cosma_split_operand(2, 1, 0, 64, None, 64, None, None) == 0
+    assert lib.cosma_split_operand(0, 1, 4, 64, 0x1000, 64, 0x2000, None) == -1  # not a split mode
     assert lib.cosma_error_stats_f32(-1, 2, None, 2, None, 2, (C.c_double * 5)(), None) == -1
     assert lib.cosma_convert_f32_f64(2, 2, None, 2, None, 2, None) == -1
 
