@@ -162,7 +162,8 @@ int cosma_workspace_release(size_t *freed_bytes);
  *   element (r, k) of the operand is X[r * ld + k] when kmajor != 0, else
  *   X[k * ld + r]; planes must hold cosma_planes_bytes(rows, k) bytes,
  *   16-byte aligned.  For A pass rows = M (kmajor = !trans_a); for B pass
- *   rows = N (kmajor = trans_b).  Asynchronous on stream. */
+ *   rows = N (kmajor = trans_b); k <= 2^31 - 32 (Kp must fit an int).
+ *   Asynchronous on stream. */
 size_t cosma_planes_bytes(int rows, int k);
 int cosma_split_operand(int mode, int kmajor, int rows, int k, const float *X, int64_t ld, void *planes,
                         void *stream);
