@@ -786,7 +786,12 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
     b.store(st + BM * BK, st + Sh::PLANE + BM * BK, tid);
   };
 
-  if constexpr (PIPE == 3) {
+  // BF16x4's guarded loads spread among the MFMAs (EDGE, PIPE 7) need about
+  // 400 registers and spill with 256 per wave; that instantiation runs the
+  // ping-pong schedule, which fits (the two compute the same products in
+  // the same order).
+  constexpr int SCHED = (PIPE == 7 && EDGE && MODE == kBF16x4) ? 4 : PIPE;
+  if constexpr (SCHED == 3) {
     // Two register sets: chunk it+2 is loaded while chunk it is computed and
     // chunk it+1 (loaded one iteration earlier) is written to LDS among the
     // MFMAs, so that copy never waits on memory.  The loop is unrolled by two
@@ -821,7 +826,7 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
       step(it + 1, la1, lb1, la, lb);
     }
     if (it < nk) step(it, la, lb, la1, lb1);
-  } else if constexpr (PIPE == 4) {
+  } else if constexpr (SCHED == 4) {
     // Ping-pong: waves in the upper half of the workgroup (one per SIMD for
     // 8 waves) copy first -- they write chunk it+1, loaded during the previous
     // iteration, then load chunk it+2 into the freed registers, then run the
@@ -860,7 +865,7 @@ __device__ __forceinline__ void gemm_tile(const GemmParams<MODE>& p, int tm, int
       }
       __syncthreads();
     }
-  } else if constexpr (PIPE == 7) {
+  } else if constexpr (SCHED == 7) {
     // Every wave copies first: it writes chunk it+1 (loaded during the
     // previous iteration) to the free stage, then runs the MFMAs of chunk it
     // with the loads of chunk it+2 spread among them, one group after each
