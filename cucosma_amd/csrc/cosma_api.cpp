@@ -80,31 +80,18 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // Heuristic tile / split choice, used when neither the caller nor the tuned
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
-//  - BF16x3 / BF16x4: the large tile -- 256x256, 8 waves of 128x64 with
-//    ping-pong copies (for BF16x4 0-14% above the copy-between-MFMAs version
-//    on eight shapes, within 3% below on two) -- whenever the output fills
-//    at least half of its tiles and the tiles, with K split into slices of at
-//    least 512, give at least half the CUs a workgroup.  K is split only when
-//    the tiles alone leave more than half the CUs idle, until every CU holds
-//    one workgroup (256x256x1048576: 256 slices of 4096, so every input byte
-//    is read from HBM once).  It beat every smaller tile from 2048^3 up, and
-//    for 4096x4096x16384 (the weight gradient of a 16384x4096x4096 layer) ran
-//    at twice the speed of 128x128.  The 256x256 tile runs over pre-split
-//    operands (planar_cfg) when its rounds of workgroups (ceil(tiles / CUs))
-//    times 5504 reach M + N: the split pass costs (M+N)*K*8 bytes of HBM
-//    traffic, the GEMM without the in-loop split saves 16-22% of each round,
-//    and both scale with K (8192^3: 2.35 ms in total against 2.64 for cfg 6;
-//    4608^2x4096, 5120^2x4096, 4352^3, 4096x6144x4096 and 4096x12288x4096,
-//    with partly filled or few rounds, 6-8% faster in both split modes;
-//    4096^3, 2048x8192x2048 and 4096x8192x4096 tie or lose, and keep cfg 6;
-//    5632^2x2048 lost 1% and is just outside).  5504 >= 2^24 / 3072, so every
-//    output with M*N/(M+N) >= 3072 also qualifies (BF16x4: 6144).  The
-//    pre-split tile is also taken when its 256x256 tiles number between half
-//    the CUs and all of them (one workgroup per CU, some CUs idle) and K is at
-//    least 512: there the 16-wave kernel's faster K loop outweighs the split
-//    pass (against the spread-load tile: 2048x4096x4096 and 3072^2x4096
-//    BF16x3 0.5-1% and BF16x4 6-8% faster, 3584^2x4096 BF16x3 3% slower and
-//    BF16x4 3% faster; at K = 256 the spread-load tile is 11-18% faster);
+//  - BF16x3 / BF16x4 large outputs (the 256x256 tiles give every CU one):
+//    the 256x256 tile unsplit, over pre-split operands (planar_cfg) when its
+//    rounds of workgroups (ceil(tiles / CUs)) times 5504 reach M + N: the
+//    split pass costs (M+N)*K*8 bytes of HBM traffic, the GEMM without the
+//    in-loop split saves 16-22% of each round, and both scale with K (8192^3:
+//    2.35 ms in total against 2.64 for cfg 6; 4608^2x4096, 5120^2x4096,
+//    4352^3, 4096x6144x4096 and 4096x12288x4096, with partly filled or few
+//    rounds, 6-8% faster in both split modes; 4096^3, 2048x8192x2048 and
+//    4096x8192x4096 tie or lose and run on the fly; 5632^2x2048 lost 1% and
+//    is just outside).  5504 >= 2^24 / 3072, so every output with
+//    M*N/(M+N) >= 3072 also qualifies (BF16x4: 6144).  The 256x256 tile beat
+//    every smaller tile from 2048^3 up.  Smaller outputs: split_medium;
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    in 16 waves of 64x32 with the next stage held in registers (four waves
 //    per SIMD; 3-5% above the 8-wave version, which itself matched the
@@ -115,15 +102,10 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    rounds of 256 become 9; +7%);
 //  - deep-K outputs (few tiles, K >= 16384): FP32 splits K over the 16-wave
 //    256x128 tile until every CU holds one workgroup (0-6% above the 8-wave
-//    ping-pong version of the tile on four deep shapes); FP64, and outputs
-//    smaller than the large tile, split a 128-row tile with a long K chunk
-//    (128x64x32, or 128x128x64 for BF16) until every CU holds two (128x64) or
-//    one (128x128x64) workgroups;
-//  - medium outputs (autotuner comparisons on one MI355X): BF16 splits K
-//    over the large tile unless that leaves CUs idle with slices of at
-//    most 512, then over 128x128 tiles toward two workgroups per CU
-//    (1024x1024x4096: 213 TFLOP/s against 160 over the large tile; at
-//    1536^2x4096 the large tile with 1024-deep slices stays ahead); FP32
+//    ping-pong version of the tile on four deep shapes); FP64, and FP32
+//    outputs smaller than the large tile, split the 128x64x32 tile until
+//    every CU holds two workgroups;
+//  - medium outputs (autotuner comparisons on one MI355X): FP32
 //    runs the large tile with K split in two when that makes about one
 //    round of workgroups (2048^3: 112 against 108 for 128x64); FP64 takes
 //    the 128x128 tile when it fills the CUs, K split in two unless its
@@ -131,11 +113,10 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    3072x2048x2048: 65 against 50 unsplit); FP32 128x64 tiles split K in
 //    two when the last round would be at most half full (3072^3: 119
 //    against 110);
-//  - otherwise 128x128 (BF16) / 128x64 tiles while they give every CU a
-//    workgroup, else 64x64; split K when the tiles give the CUs fewer than
-//    two workgroups each, aiming at three, with K slices at least 256 deep
-//    (1024^3 with 64x64 tiles: FP32 split 3 ran in 0.038 ms against 0.050
-//    unsplit, BF16x3 split 3 in 0.031 against 0.035 with split 2; cache
+//  - otherwise 128x64 tiles while they give every CU a workgroup, else
+//    64x64; split K when the tiles give the CUs fewer than two workgroups
+//    each, aiming at three, with K slices at least 256 deep (1024^3 with
+//    64x64 tiles: FP32 split 3 ran in 0.038 ms against 0.050 unsplit; cache
 //    flushed, beta 0.5).
 struct Choice {
   int cfg, split;
@@ -156,13 +137,13 @@ constexpr int planar_cfg(int mode) { (void)mode; return 8; }
 // 0.221), BF16x4 0-3% on seven of nine, 256x256x1048576 12% (0.544 against
 // 0.617), 2048^2x16384 1.7% slower.
 constexpr int large_cfg(int mode) { return mode == 0 ? 7 : (mode == 1 ? 7 : 12); }
-constexpr int medium_cfg(int mode) { return split_mode(mode) ? 0 : 2; }
-// Deep-K tile: the large tile when the output fills it, else a 128-row tile
-// with a long K chunk (128x64x32 for FP32/FP64, 128x128x64 for BF16).
+constexpr int medium_cfg(int mode) { return split_mode(mode) ? 11 : 2; }
+// FP32 / FP64 deep-K tile: the large FP32 tile when the output fills it,
+// else 128x64x32.
 constexpr int deep_cfg(int mode, int m, int n) {
-  const int big = mode == 0 ? 7 : (mode == 1 ? 2 : 12);
+  const int big = mode == 0 ? 7 : 2;
   if (m >= kDims[mode][big].bm && n >= kDims[mode][big].bn) return big;
-  return split_mode(mode) ? 5 : 2;
+  return 2;
 }
 constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   int max_by_depth = k / min_depth;
@@ -170,6 +151,79 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   if (split > max_split) split = max_split;
   return split < 1 ? 1 : split;
 }
+// BF16x3 / BF16x4 outputs whose 256x256 tiles do not give every CU a
+// workgroup.  Rules read off tuner tables of three shape sets on one MI355X
+// (cache flushed; every tile configuration at K splits 1, 2, 4, ...,
+// floor(CUs / tiles) and floor(2 CUs / tiles)): 94 shapes per mode with M = N
+// from 384 to 3072 and K from 512 to 32768 plus ten rectangular ones; 58 with
+// M = N from 448 to 3328 (not multiples of 256) and K from 768 to 24576 plus
+// ten rectangular ones; 45 with M = N from 320 to 3200 and K from 640 to
+// 20480 plus ten rectangular ones.  Against the fastest candidate of each
+// shape these picks are 1.3% / 1.9% / 2.3% slower in the geometric mean for
+// BF16x3 and 0.7% / 1.1% / 4.5% for BF16x4 (up to 1.25x on a few shapes);
+// the rules they replace were 6.6% / 7.8% / 9.4% and 10.0% / 13.6% / 14.7%
+// slower (up to 2.2x).
+//  - 256x256 spread-load tile (cfg 12): K split for one round of workgroups
+//    (floor(CUs / tiles)) or for two (floor(2 CUs / tiles)), whichever makes
+//    rounds * ceil(K / split) + D * split smaller; D (400 BF16x3, 150 BF16x4)
+//    stands for the split-K partials' cost in units of K (2560^2: two rounds,
+//    split 5, win from K = 16384 in BF16x3 and from 4096 in BF16x4).  A split
+//    needs K >= 2048 and slices at least 512 deep in BF16x3 (400 when 128x128
+//    tiles fill under 90% of their slots: 1792^2x2048 split 5, 8% faster than
+//    128x128 split 2) or 640 in BF16x4, whose 128x128 tile is as fast per
+//    output as this one (2048^2x2048: 128x128 split 2 9% faster than 256x256
+//    split 4); a two-round split that falls short of that gives way to the
+//    one-round split.  Taken when its workgroups fill three quarters of the
+//    CUs, BF16x4 has K >= 2048, and its useful share of the work (output over
+//    tile area, times the filled share of its rounds) is at least the 128x128
+//    tiles' (1152^2 to 1664^2: 81-86% against 95-99%, 128x128 tiles 3-26%
+//    faster; 1856^2: 82% against 82%, this tile 6-27% faster in BF16x3).
+//  - One round of unsplit 256x256 workgroups filling at least half the CUs,
+//    tiles covering the output at least 88%, K >= 512 (BF16x4: 1024): the
+//    pre-split tile (3500^2x3000 16-17% faster than cfg 12; 3072^2x1024
+//    BF16x3 14% faster than 128x128 tiles), except that BF16x4 with K <= 2048
+//    takes unsplit 128x128 tiles when they fill 60% of their slots
+//    (3328^2x1536: 7% faster).
+//  - 128x128 (cfg 11: 8 waves, two workgroups per CU): K split toward two
+//    workgroups per CU with slices at least 192 deep (from K = 1024; when
+//    the tiles alone give every CU a workgroup, from K = 2048), taken when
+//    that fills three quarters of the slots or every CU has a tile.  It ran
+//    0-30% faster than the 4-wave 128x128 tile (cfg 0) in BF16x4 and within
+//    3% of it in BF16x3.
+//  - Otherwise 64x64 (cfg 1), K split toward three workgroups per CU with
+//    slices at least 128 deep while the tiles number fewer than twice the
+//    CUs (fewer than the CUs below K = 1024; 384^2x512: split 4, 1.7x faster
+//    than unsplit).
+constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
+  const bool x4 = mode == 3;
+  const int64_t lt = tiles_of(mode, 12, m, n);
+  const int64_t t11 = tiles_of(mode, 11, m, n);
+  const int64_t t1 = tiles_of(mode, 1, m, n);
+  int s11 = 1;
+  if ((t11 >= cus && k >= 2048) || (t11 < cus && k >= 1024)) s11 = clamp_split((int)(2 * cus / t11), k, 192, cus);
+  const int64_t w11 = t11 * s11;
+  const int64_t slots11 = 2 * (int64_t)cus * cdiv(w11, 2 * (int64_t)cus);  // whole rounds of two per CU
+  const int64_t d_units = x4 ? 150 : 400;
+  const int sa = (int)(cus / lt > 1 ? cus / lt : 1);
+  const int sb = (int)(2 * cus / lt > 1 ? 2 * cus / lt : 1);
+  auto cost = [&](int s) { return (int64_t)cdiv(lt * s, cus) * cdiv(k, s) + d_units * s; };
+  const int depth = x4 ? 640 : (10 * w11 < 9 * slots11 ? 400 : 512);
+  int s12 = cost(sa) <= cost(sb) ? sa : sb;
+  if (s12 == sb && s12 > 1 && !(k >= 2048 && k / s12 >= depth)) s12 = sa;
+  const bool cover_ok = 25 * (int64_t)m * n >= 22 * lt * 65536;
+  const bool x4_c11 = x4 && 2 * t11 >= 3 * (int64_t)cus && k <= 2048 && 5 * w11 >= 3 * slots11;
+  if (2 * lt >= cus && s12 == 1 && k >= (x4 ? 1024 : 512) && !x4_c11 && cover_ok)
+    return Choice{planar_cfg(mode), 1};
+  const bool deep_ok = s12 == 1 ? (!x4 || k >= 2048) : (k >= 2048 && k / s12 >= depth);
+  // Useful work share, 256x256 against 128x128: M*N*s12 / (65536 * CUs *
+  // rounds12) >= M*N*s11 / (16384 * slots11), with M*N cancelled.
+  const bool share_ok = (int64_t)s12 * slots11 >= 4 * (int64_t)s11 * cus * cdiv(lt * s12, cus);
+  if (share_ok && 4 * lt * s12 >= 3 * (int64_t)cus && !x4_c11 && deep_ok) return Choice{12, s12};
+  if (2 * w11 >= 3 * (int64_t)cus || t11 >= cus) return Choice{11, s11};
+  if (t1 >= 2 * cus || (t1 >= cus && k < 1024)) return Choice{1, 1};
+  return Choice{1, clamp_split(cdiv(3 * (int64_t)cus, t1), k, 128)};
+}
+
 // ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A
 // in their loaders; the pre-split pass does it once, so with a transposed A and
 // a deep K the pre-split tile is taken even at one round of workgroups (the
@@ -183,29 +237,22 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
   const int lt = tiles_of(mode, cfg, m, n);
   if (split_mode(mode)) {
     const int64_t covered = (int64_t)lt * kDims[mode][cfg].bm * kDims[mode][cfg].bn;
-    const int split = lt >= cus / 2 ? 1 : clamp_split(cus / lt, k, 512, cus);
-    // K is split over the large tile while that gives half the CUs a
-    // workgroup, unless the slices end up at most 512 deep and the CUs are
-    // still not all busy (then 128x128 tiles below do better).
-    const bool thin = split > 1 && (int64_t)lt * split < cus && k / split <= 512;
-    if (2 * (int64_t)m * n >= covered && (int64_t)lt * split >= cus / 2 && !thin) {
-      // Large outputs, and outputs whose tiles leave CUs idle: split the
-      // operands once and copy planes into LDS.
-      // Against the spread-load on-the-fly tile (cfg 12; cache flushed, the two
-      // orders alternated on one box): nothing pre-splits below K = 512, where
-      // the extra launch outweighs the faster loop (3072^2x256: cfg 12 18% / 11%
-      // faster in BF16x3 / BF16x4; 3072^2x4096: the pre-split tile 0.5% / 6%,
-      // one partly filled round); A stored transposed from K
-      // = 4096 on (4096^2x1024: cfg 12 9% / 3% faster); and the rounds rule's
-      // constant is 5504 for BF16x3 and 6144 for BF16x4, whose pre-split kernel
-      // gains more per round (5632^2x2048 and 4096x8192x4096: BF16x4 pre-split
-      // 3-4% faster, BF16x3 within 1.6% either way).
+    if (lt >= cus && 2 * (int64_t)m * n >= covered) {
+      // Large outputs: split the operands once and copy planes into LDS when
+      // the rounds rule holds.  Against the spread-load on-the-fly tile (cfg
+      // 12; cache flushed, the two orders alternated on one box): nothing
+      // pre-splits below K = 512, where the extra launch outweighs the faster
+      // loop; A stored transposed from K = 4096 on (4096^2x1024: cfg 12 9% /
+      // 3% faster); and the rounds rule's constant is 5504 for BF16x3 and 6144
+      // for BF16x4, whose pre-split kernel gains more per round (5632^2x2048
+      // and 4096x8192x4096: BF16x4 pre-split 3-4% faster, BF16x3 within 1.6%
+      // either way).
       const int64_t per_round = mode == 3 ? 6144 : 5504;
-      if (split == 1 && k >= 512 &&
-          (lt < cus || (ta && k >= 4096) || (int64_t)cdiv(lt, cus) * per_round >= (int64_t)m + n))
+      if (k >= 512 && ((ta && k >= 4096) || (int64_t)cdiv(lt, cus) * per_round >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
-      return Choice{cfg, split};
+      return Choice{cfg, 1};
     }
+    return split_medium(mode, m, n, k, cus);
   } else if (lt >= 2 * cus) {
     const int rem = lt % cus;
     const bool even_out = mode == 0 && rem > 0 && rem <= cus / 2 && lt < 8 * cus && k >= 2048;
@@ -232,17 +279,7 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
     return Choice{3, (t3 % cus == 0 && k < 4096) ? 1 : 2};
   }
   cfg = medium_cfg(mode);
-  if (tiles_of(mode, cfg, m, n) < cus) {
-    // BF16: 128x128 with K split (a power of two) toward two workgroups per
-    // CU when that fills every CU with slices at least 512 deep; otherwise
-    // 64x64.
-    const int t0 = tiles_of(mode, cfg, m, n);
-    int want = 1;
-    while (want * 2 * t0 <= 2 * cus) want *= 2;
-    const int s0 = clamp_split(want, k, 512);
-    if (split_mode(mode) && (int64_t)t0 * s0 >= cus) return Choice{cfg, s0};
-    cfg = 1;  // 64x64
-  }
+  if (tiles_of(mode, cfg, m, n) < cus) cfg = 1;  // 64x64
   const int tiles = tiles_of(mode, cfg, m, n);
   int split = 1;
   if (tiles < 2 * cus && k >= 1024) {
@@ -502,7 +539,10 @@ void resolve_choice(const cosma_gemm_desc* d, int* cfg, int* split, int* group_m
 // process ends.  The array's host-side name is not a device address: its
 // address on a device is taken once per device (scratch_ptr) with
 // hipGetSymbolAddress, and only that address is passed to kernels.
-constexpr size_t kScratchBytes = (size_t)64 << 20;  // 256x256 outputs with 256 FP32 K slices
+// 128 MiB: the largest split-K partials the heuristic asks for on 256 CUs
+// (two rounds of 256x256 FP32 output tiles), so every default choice but
+// the pre-split operand planes fits.
+constexpr size_t kScratchBytes = (size_t)128 << 20;
 static __device__ __align__(256) char scratch[kScratchBytes];
 std::mutex g_scratch_mu;
 std::vector<void*> g_scratch_dev;  // per device: scratch's address there
@@ -1691,16 +1731,31 @@ int cosma_autotune_ex(const cosma_gemm_desc* d, int warmup, int reps, size_t flu
     if (status == 0) set_time(r, ms);
     all.push_back(r);
   };
-  // Pass 1: tile configuration x split-K at the default group size.
+  // Pass 1: tile configuration x split-K at the default group size.  Splits:
+  // the powers of two, the two that give every CU one or two workgroups
+  // exactly (floor(CUs / tiles), floor(2 CUs / tiles)), and the heuristic's
+  // own choice, so the tuner never reports a slower pick than the default.
   const int cfg_lo = d->tile_cfg >= 0 ? d->tile_cfg : 0;
   const int cfg_hi = d->tile_cfg >= 0 ? d->tile_cfg + 1 : kNumCfg[d->mode];
   const int gm0 = d->group_m > 0 ? d->group_m : 8;
+  const Choice hc = heuristic(d->mode, d->m, d->n, d->k, d->trans_a, topo.cus);
   for (int cfg = cfg_lo; cfg < cfg_hi; ++cfg) {
     const TileDims td = kDims[d->mode][cfg];
     const int tiles = tiles_of(d->mode, cfg, d->m, d->n);
-    for (int split = 1; split <= 1024; split *= 2) {
+    std::vector<int> splits;
+    for (int split = 1; split <= 1024; split *= 2) splits.push_back(split);
+    splits.push_back(topo.cus / tiles);
+    splits.push_back(2 * topo.cus / tiles);
+    if (cfg == hc.cfg) splits.push_back(hc.split);
+    std::vector<int> tried;
+    for (int split : splits) {
+      if (split < 1 || std::find(tried.begin(), tried.end(), split) != tried.end()) continue;
+      tried.push_back(split);
       if (d->split_k > 0 && split != d->split_k) continue;
-      if (split > 1 && (split * 2 * td.bk > d->k || (int64_t)tiles * split > 16 * (int64_t)topo.cus)) continue;
+      const bool is_default = cfg == hc.cfg && split == hc.split;
+      if (split > 1 && !is_default &&
+          (split * 2 * td.bk > d->k || (int64_t)tiles * split > 16 * (int64_t)topo.cus))
+        continue;
       try_one(cfg, split, gm0);
     }
     if (d->split_k > 1024) try_one(cfg, d->split_k, gm0);

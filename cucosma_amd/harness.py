@@ -28,6 +28,8 @@ Configurations:
   autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
   schedules the two on-the-fly 256x256 BF16 schedules (ping-pong cfg 6, spread
            loads cfg 12) on five shapes, orders alternated round by round
+  heuristic the library's default BF16 tile and split against the autotuner's
+           fastest candidate on three shape sets of 94, 58 and 45 shapes per mode
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
 """
@@ -392,9 +394,62 @@ def cfg_schedules(cc, torch, args, emit):
             del A, B, Cm
 
 
+def heuristic_shape_sets():
+    """Three BF16 shape sets whose 256x256 tiles mostly leave CUs idle: M = N
+    from 384 to 3072 (multiples of 256 and 128) with K from 512 to 32768;
+    M = N from 448 to 3328 (not multiples of 256) with K from 768 to 24576;
+    M = N from 320 to 3200 with K from 640 to 20480; ten rectangular shapes
+    each."""
+    a = [(m, m, k) for m in (384, 512, 640, 768, 1024, 1280, 1536, 1792, 2048, 2304, 2560, 3072)
+         for k in (512, 1024, 2048, 4096, 8192, 16384, 32768)]
+    a += [(1024, 2048, 4096), (2048, 1024, 8192), (512, 2048, 8192), (2048, 512, 8192), (1024, 4096, 2048),
+          (4096, 1024, 2048), (768, 3072, 4096), (3072, 768, 4096), (1536, 3072, 2048), (512, 4096, 4096)]
+    b = [(m, m, k) for m in (448, 896, 1152, 1408, 1664, 2176, 2816, 3328) for k in (768, 1536, 3072, 6144, 12288, 24576)]
+    b += [(1000, 3000, 5000), (700, 5000, 3000), (4000, 600, 8000), (1500, 1500, 7000), (2900, 2900, 1100),
+          (3000, 1000, 2000), (600, 600, 20000), (1200, 2400, 600), (3500, 3500, 3000), (2200, 2200, 9000)]
+    c = [(m, m, k) for m in (320, 576, 960, 1216, 1856, 2432, 3200) for k in (640, 2560, 5120, 10240, 20480)]
+    c += [(800, 2000, 6000), (2000, 800, 1500), (1300, 2600, 3500), (300, 3000, 12000), (3100, 1700, 900),
+          (1750, 1750, 2500), (2600, 2600, 4500), (500, 1500, 30000), (2300, 1100, 16000), (3600, 3000, 1800)]
+    return {"A": a, "B": b, "C": c}
+
+
+def cfg_heuristic(cc, torch, args, emit):
+    """The library's default (tile, split) for BF16 shapes against the
+    autotuner's fastest candidate: every tile configuration at K splits 1, 2,
+    4, ..., floor(CUs / tiles), floor(2 CUs / tiles) and the default's own,
+    cache flushed, the four fastest re-timed in alternating order (the
+    default's time is a single pass of --runs/20 calls unless it is among
+    them).  One line per shape (default time / fastest), then per set and
+    mode the geometric mean, the largest ratio and the count above 1.10."""
+    from . import autotune
+
+    reps = max(3, args.runs // 20)
+    for mode in ("bf16x3", "bf16x4"):
+        mi = cc._lib.MODES[mode]
+        for name, shapes in heuristic_shape_sets().items():
+            ratios = []
+            for (M, N, K) in shapes:
+                rec = autotune.tune(mode, M, N, K, warmup=1, reps=reps)
+                cfg, split = cc.heuristic(mi, M, N, K)
+                ok = [c for c in rec["candidates"] if c["status"] == 0]
+                best = ok[0]
+                mine = [c for c in ok if c["tile_cfg"] == cfg and c["split_k"] == split and c["group_m"] == 8]
+                if not mine:
+                    continue
+                ratio = mine[0]["ms"] / best["ms"]
+                ratios.append(ratio)
+                emit(dict(config="heuristic", set=name, mode=mode, M=M, N=N, K=K, tile_cfg=cfg, split_k=split,
+                          ms=mine[0]["ms"], best_tile_cfg=best["tile_cfg"], best_split_k=best["split_k"],
+                          best_group_m=best["group_m"], best_ms=best["ms"], ratio=ratio))
+            if ratios:
+                emit(dict(config="heuristic_summary", set=name, mode=mode, shapes=len(ratios),
+                          geomean=float(np.exp(np.mean(np.log(ratios)))), max=float(max(ratios)),
+                          above_1_10=int(sum(r > 1.10 for r in ratios))))
+
+
 CONFIGS = dict(plain=cfg_plain, square=cfg_square, flat=cfg_flat, dgemm=cfg_dgemm,
                grouped=cfg_grouped, grouped_tiles=cfg_grouped_tiles, autograd=cfg_autograd, epilogue=cfg_epilogue,
-               schedules=cfg_schedules)
+               schedules=cfg_schedules, heuristic=cfg_heuristic)
 
 
 def main(argv=None) -> int:

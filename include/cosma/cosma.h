@@ -109,7 +109,7 @@ int cosma_gemm_ex(const cosma_gemm_desc *d);
  * capture, serving loops, harnesses that forbid hipMalloc).  No counterpart
  * in the reference.  Allocates nothing: split-K partials and the planes of
  * operands split inside the call live in a static device array of
- * cosma_static_scratch_bytes() bytes (64 MiB) per GPU, held until the
+ * cosma_static_scratch_bytes() bytes (128 MiB) per GPU, held until the
  * process ends; its address on a device is looked up on the first call on
  * that device.  d->workspace must be NULL (an installed workspace is not
  * used).  Calls that need more return COSMA_STATUS_SCRATCH, except that the

@@ -98,10 +98,45 @@ def test_heuristic_round_rules():
         # BF16x4's rounds constant is 6144: two rounds reach M + N = 11264.
         assert h(mode, 5632, 5632, 2048) == ((12, 1) if mode == 2 else (8, 1))
         assert h(mode, 4096, 8192, 4096) == ((12, 1) if mode == 2 else (8, 1))
-        # One partly filled round pre-splits only from K = 512 on.
+        # One partly filled round (144 tiles): the pre-split tile from K = 512
+        # (BF16x4: 1024) on, 128x128 tiles below.
         assert h(mode, 3072, 3072, 1024) == (8, 1)
-        assert h(mode, 3072, 3072, 256) == (12, 1)
+        assert h(mode, 3072, 3072, 256) == (11, 1)
         assert h(mode, 8192, 8192, 8192) == (8, 1)
+
+
+def test_heuristic_split_medium_rules():
+    """BF16 outputs whose 256x256 tiles leave CUs idle (host code, nominal
+    256 CUs): each case is a rule of split_medium, and each pick was the
+    tuner's fastest configuration for the shape (cache flushed, one MI355X)."""
+    import cucosma_amd
+
+    h = cucosma_amd.heuristic
+    # 256x256 split for one round of workgroups (64 tiles x 4; 49 x 5 with
+    # 409-deep slices, 128x128 tiles filling 77% of their slots).
+    assert h(2, 2048, 2048, 2048) == (12, 4)
+    assert h(2, 1792, 1792, 2048) == (12, 5)
+    assert h(2, 1024, 1024, 16384) == (12, 16)
+    # ... or for two rounds when one leaves many CUs idle and K is deep.
+    assert h(2, 2560, 2560, 8192) == (12, 2)
+    assert h(2, 2560, 2560, 32768) == (12, 5)
+    assert h(3, 3072, 3072, 4096) == (12, 3)
+    # BF16x4 takes 128x128 tiles where they are as fast and fill better.
+    assert h(3, 2048, 2048, 2048) == (11, 2)
+    assert h(3, 2560, 2560, 2048) == (11, 1)
+    assert h(3, 1024, 1024, 8192) == (11, 8)
+    assert h(3, 3328, 3328, 1536) == (11, 1)
+    # 256x256 tiles covering under 88% of the output: 128x128 tiles, K split
+    # toward two workgroups per CU.
+    assert h(2, 384, 384, 16384) == (11, 56)
+    assert h(3, 640, 640, 16384) == (11, 20)
+    assert h(3, 1408, 1408, 6144) == (11, 4)
+    # One unsplit round filling half the CUs or more: the pre-split tile.
+    assert h(2, 3500, 3500, 3000) == (8, 1)
+    assert h(3, 2900, 2900, 1100) == (8, 1)
+    # 64x64: K split toward three workgroups per CU, slices at least 128 deep.
+    assert h(2, 384, 384, 512) == (1, 4)
+    assert h(3, 1024, 1024, 1024) == (1, 3)
 
 
 def test_transposed_a_takes_the_presplit_tile():

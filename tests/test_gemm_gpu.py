@@ -322,25 +322,27 @@ def test_presplit_bitwise_equals_on_the_fly(cc, dev, mode, ta, tb):
 
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
 def test_heuristic_presplit_for_underfilled_tiles(cc, dev, mode):
-    """Outputs whose 256x256 tiles fill between half the CUs and all of them
-    take the pre-split configuration by default from K = 512 on (library
-    workspace for the planes); the result is bitwise the on-the-fly 256x256
-    tile's."""
+    """3072^2 x 1024 (144 256x256 tiles, one partly filled round) takes the
+    pre-split configuration by default (library workspace for the planes),
+    K = 256 128x128 tiles; either result is bitwise the on-the-fly 256x256
+    tile's unsplit (the same products in the same order)."""
     planars, fly = PRESPLIT[mode]
     same = fly[0]
     M, N, K = 3072, 3072, 1024
-    assert cc.heuristic(cc._lib.MODES[mode], M, N, 256)[0] == same
+    assert cc.heuristic(cc._lib.MODES[mode], M, N, 256) == (11, 1)
     cfg, split = cc.heuristic(cc._lib.MODES[mode], M, N, K)
     assert cfg in planars and split == 1
     assert cc.heuristic(cc._lib.MODES[mode], 4096, 4096, 4096)[0] == same
     rng = np.random.default_rng(403)
     A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
     B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
-    for ta in (False, True):
-        Ad, Bd = _t(A.T if ta else A, dev), _t(B, dev)
-        got = cc.gemm(Ad, Bd, mode=mode, trans_a=ta).cpu().numpy()
-        want = cc.gemm(Ad, Bd, mode=mode, trans_a=ta, tile_cfg=same, split_k=1).cpu().numpy()
-        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), ta
+    for k in (K, 256):
+        Ak, Bk = np.ascontiguousarray(A[:, :k]), np.ascontiguousarray(B[:k])
+        for ta in (False, True):
+            Ad, Bd = _t(Ak.T if ta else Ak, dev), _t(Bk, dev)
+            got = cc.gemm(Ad, Bd, mode=mode, trans_a=ta).cpu().numpy()
+            want = cc.gemm(Ad, Bd, mode=mode, trans_a=ta, tile_cfg=same, split_k=1).cpu().numpy()
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (k, ta)
 
 
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])

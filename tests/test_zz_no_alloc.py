@@ -55,7 +55,7 @@ def _both(cc, mode, m, n, k, seed, beta=0.0):
 
 
 def test_static_scratch_matches_gemm_ex(cc, dev):
-    assert cc.lib.cosma_static_scratch_bytes() == 64 << 20
+    assert cc.lib.cosma_static_scratch_bytes() == 128 << 20
     # FP32 deep K: split-K partials in the array (32 slices of 512x512 = 32 MiB),
     # then a smaller split call reusing it.
     for (mode, m, n, k) in (("fp32", 512, 512, 65536), ("fp32", 128, 256, 16384)):
@@ -63,9 +63,15 @@ def test_static_scratch_matches_gemm_ex(cc, dev):
         assert split > 1, (m, n, k)
         ref, got = _both(cc, mode, m, n, k, seed=m + k, beta=0.5)
         assert torch.equal(ref, got), (mode, m, n, k)
+    # BF16x4 3072^2x4096: the 256x256 tile with K split for two rounds of
+    # workgroups (3 x 36 MiB of partials, more than half the array).
+    m, n, k = 3072, 3072, 4096
+    assert cc.heuristic(cc.MODE_BF16X4, m, n, k) == (12, 3)
+    ref, got = _both(cc, "bf16x4", m, n, k, seed=5, beta=0.5)
+    assert torch.equal(ref, got)
     # BF16x3 on pre-split operands: planes in the array (48 MiB), then a
     # smaller pre-split call (24 MiB).
-    for (m, n, k) in ((4096, 2048, 2048), (2048, 4096, 1024)):
+    for (m, n, k) in ((3072, 3072, 2048), (3072, 3072, 1024)):
         tiles = cc.tile_configs(cc.MODE_BF16X3)
         assert tiles[cc.heuristic(cc.MODE_BF16X3, m, n, k)[0]]["pipe"] == 5
         ref, got = _both(cc, "bf16x3", m, n, k, seed=m)
@@ -73,15 +79,15 @@ def test_static_scratch_matches_gemm_ex(cc, dev):
 
 
 def test_static_scratch_limits(cc, dev):
-    # Planes that do not fit (8192x8192x2048: 128 MiB): the on-the-fly tile,
+    # Planes that do not fit (8192x8192x4096: 256 MiB): the on-the-fly tile,
     # bitwise the pre-split result of cosma_gemm_ex.
-    ref, got = _both(cc, "bf16x3", 8192, 8192, 2048, seed=3)
+    ref, got = _both(cc, "bf16x3", 8192, 8192, 4096, seed=3)
     assert torch.equal(ref, got)
     # Partials that do not fit: refused before any launch, C untouched.
     A = torch.rand((512, 4096), device="cuda")
     B = torch.rand((4096, 512), device="cuda")
     Cm = torch.full((512, 512), 7.0, device="cuda")
-    assert cc.lib.cosma_gemm_ex_static(C.byref(_desc(cc, "fp32", A, B, Cm, split_k=128))) == -5
+    assert cc.lib.cosma_gemm_ex_static(C.byref(_desc(cc, "fp32", A, B, Cm, split_k=256))) == -5
     ws = torch.empty(1 << 20, dtype=torch.uint8, device="cuda")
     assert cc.lib.cosma_gemm_ex_static(C.byref(_desc(cc, "fp32", A, B, Cm, split_k=2, workspace=ws))) == -1
     torch.cuda.synchronize()
