@@ -80,6 +80,9 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // Heuristic tile / split choice, used when neither the caller nor the tuned
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
+//  - FP32 outputs in the range of its fitted cost model (fewer 256x128 tiles
+//    than twice the CUs, both sides >= 256, 512 <= K <= 32768): fp32_medium;
+//    the FP32 rules below cover the rest;
 //  - BF16x3 / BF16x4 large outputs (the 256x256 tiles give every CU one):
 //    the 256x256 tile unsplit, over pre-split operands (planar_cfg) when its
 //    rounds of workgroups (ceil(tiles / CUs)) times 5504 reach M + N: the
@@ -224,6 +227,74 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   return Choice{1, clamp_split(cdiv(3 * (int64_t)cus, t1), k, 128)};
 }
 
+// FP32 outputs whose 256x128 tiles number fewer than twice the CUs, both
+// sides at least 256, 512 <= K <= 32768: the (tile, split) of least modelled
+// time.  The model is fitted (least squares on relative error) to full tuner
+// tables of the three BF16 shape sets run in FP32 on one MI355X (197 shapes,
+// cache flushed, every tile configuration at K splits 1, 2, 4, ...,
+// floor(CUs / tiles) and floor(2 CUs / tiles)); its time for a tile of t
+// tiles split s ways over slots workgroups per CU is, in microseconds,
+//   round_k * R * ks + load_k * L * ks + round * R + base
+//     + edge_k * R * ks (M, N or K not a multiple of the tile: guarded loads)
+//     + 205.432 us/GB * (2s + 1) * M * N * 4 bytes (split-K partials, s > 1),
+// with R = ceil(t*s / (slots*CUs)) rounds, L = ceil(t*s / CUs) workgroups on
+// the busiest CU and ks the K slice in thousands.  Its error is 4.4% RMS over
+// all candidates; its picks are 1.4% / 1.3% / 1.5% slower than the fastest
+// candidate in the geometric mean on the three sets (fitted on the first two
+// only, 1.8% on the third), where the rules they replace were 17-19% slower
+// (101 of 197 shapes beyond 10%, up to 2.2x: splits rounded up past one round
+// of workgroups, no split below K = 1024).
+struct CostRow {
+  int cfg, bm, bn, slots;
+  double round_k, load_k, round, base, edge_k;
+};
+constexpr CostRow kFp32Cost[] = {
+    {7, 256, 128, 1, 59.8401, 59.8401, 3.4542, 10.3971, 8.04107},
+    {8, 128, 256, 1, 64.4791, 64.4791, 2.44158, 10.642, 3.68721},
+    {9, 128, 128, 1, 33.1662, 33.1662, 1.00551, 10.5337, 6.97167},
+    {2, 128, 64, 3, 14.1125, 27.9115, 0.276966, 12.3699, 12.537},
+    {1, 64, 64, 4, 21.8173, 11.2695, 0.983808, 9.25921, 15.1252},
+};
+constexpr double kPartialUsPerGB = 205.432;
+constexpr bool fp32_model_applies(int m, int n, int k, int cus) {
+  return tiles_of(0, 7, m, n) < 2 * cus && m >= 256 && n >= 256 && k >= 512 && k <= 32768;
+}
+constexpr Choice fp32_medium(int m, int n, int k, int cus) {
+  Choice best{7, 1};
+  double best_us = 0.0;
+  bool have = false;
+  for (const CostRow& c : kFp32Cost) {
+    const int64_t t = (int64_t)cdiv(m, c.bm) * cdiv(n, c.bn);
+    const bool edge = m % c.bm || n % c.bn || k % 32;
+    // Splits 1, 2, 4, ..., 64 and those filling one or two workgroups per CU,
+    // in ascending order (ties keep the first, smallest split).
+    int cand[9] = {1, 2, 4, 8, 16, 32, 64, (int)(cus / t > 1 ? cus / t : 1), (int)(2 * cus / t > 1 ? 2 * cus / t : 1)};
+    for (int i = 1; i < 9; ++i)
+      for (int j = i; j > 0 && cand[j] < cand[j - 1]; --j) {
+        const int x = cand[j];
+        cand[j] = cand[j - 1];
+        cand[j - 1] = x;
+      }
+    for (int i = 0; i < 9; ++i) {
+      const int s = cand[i];
+      if (i > 0 && s == cand[i - 1]) continue;
+      if (s > 1 && cdiv(k, s) < 64) continue;
+      const int64_t wg = t * s;
+      const double R = (double)cdiv(wg, (int64_t)c.slots * cus);
+      const double L = (double)cdiv(wg, cus);
+      const double ks = (double)((int64_t)cdiv(cdiv(k, s), 32) * 32) / 1000.0;
+      double us = c.round_k * R * ks + c.load_k * L * ks + c.round * R + c.base + (edge ? c.edge_k * R * ks : 0.0);
+      if (s > 1) us += kPartialUsPerGB * (2.0 * s + 1.0) * (double)m * (double)n * 4.0 / 1e9;
+      if (!have || us < best_us) {
+        best_us = us;
+        best = Choice{c.cfg, s};
+        have = true;
+      }
+    }
+  }
+  return best;
+}
+
 // ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A
 // in their loaders; the pre-split pass does it once, so with a transposed A and
 // a deep K the pre-split tile is taken even at one round of workgroups (the
@@ -233,6 +304,7 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
 constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
   // An empty output launches nothing; a non-positive CU count is not a device.
   if (m <= 0 || n <= 0 || cus <= 0) return Choice{medium_cfg(mode), 1};
+  if (mode == 0 && fp32_model_applies(m, n, k, cus)) return fp32_medium(m, n, k, cus);
   int cfg = large_cfg(mode);
   const int lt = tiles_of(mode, cfg, m, n);
   if (split_mode(mode)) {

@@ -28,8 +28,9 @@ Configurations:
   autograd X[16384x4096] W[4096x4096], forward + backward, bf16x3
   schedules the two on-the-fly 256x256 BF16 schedules (ping-pong cfg 6, spread
            loads cfg 12) on five shapes, orders alternated round by round
-  heuristic the library's default BF16 tile and split against the autotuner's
-           fastest candidate on three shape sets of 94, 58 and 45 shapes per mode
+  heuristic the library's default tile and split against the autotuner's fastest
+           candidate on three shape sets of 94, 58 and 45 shapes (BF16x3 and BF16x4,
+           or --modes)
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
 """
@@ -414,8 +415,8 @@ def heuristic_shape_sets():
 
 
 def cfg_heuristic(cc, torch, args, emit):
-    """The library's default (tile, split) for BF16 shapes against the
-    autotuner's fastest candidate: every tile configuration at K splits 1, 2,
+    """The library's default (tile, split) against the autotuner's fastest
+    candidate (BF16x3 and BF16x4, or --modes): every tile configuration at K splits 1, 2,
     4, ..., floor(CUs / tiles), floor(2 CUs / tiles) and the default's own,
     cache flushed, the four fastest re-timed in alternating order (the
     default's time is a single pass of --runs/20 calls unless it is among
@@ -424,7 +425,7 @@ def cfg_heuristic(cc, torch, args, emit):
     from . import autotune
 
     reps = max(3, args.runs // 20)
-    for mode in ("bf16x3", "bf16x4"):
+    for mode in (args.modes.split(",") if args.modes else ("bf16x3", "bf16x4")):
         mi = cc._lib.MODES[mode]
         for name, shapes in heuristic_shape_sets().items():
             ratios = []
@@ -464,6 +465,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-search", action="store_true",
                     help="grouped: time only the default plan (skip the grouped autotuner)")
     ap.add_argument("--grouped-modes", help="grouped: comma-separated modes (default all four)")
+    ap.add_argument("--modes", help="heuristic: comma-separated modes (default bf16x3,bf16x4)")
     args = ap.parse_args(argv)
 
     import torch

@@ -105,6 +105,25 @@ def test_heuristic_round_rules():
         assert h(mode, 8192, 8192, 8192) == (8, 1)
 
 
+def test_heuristic_fp32_cost_model():
+    """FP32 outputs whose 256x128 tiles number fewer than twice the CUs (both
+    sides >= 256, 512 <= K <= 32768) take the (tile, split) of least modelled
+    time (host code, nominal 256 CUs); each pick below was the tuner's
+    fastest configuration for the shape (cache flushed, one MI355X)."""
+    import cucosma_amd
+
+    h = cucosma_amd.heuristic
+    assert h(0, 1536, 1536, 16384) == (7, 7)    # 72 tiles: two rounds at 98%, not split 4 (56%)
+    assert h(0, 2304, 2304, 8192) == (7, 3)
+    assert h(0, 3072, 3072, 16384) == (7, 8)    # 288 tiles: 9 rounds at 100%
+    assert h(0, 1792, 1792, 2048) == (7, 5)
+    assert h(0, 3072, 3072, 512) == (1, 1)      # short K: 64x64 tiles
+    assert h(0, 384, 384, 512) == (1, 8)        # K split even below K = 1024
+    # Outside the fitted range the rules stay: large outputs, the flat shape.
+    assert h(0, 8192, 8192, 8192) == (7, 1)
+    assert h(0, 256, 256, 1 << 20)[0] == 7
+
+
 def test_heuristic_split_medium_rules():
     """BF16 outputs whose 256x256 tiles leave CUs idle (host code, nominal
     256 CUs): each case is a rule of split_medium, and each pick was the
