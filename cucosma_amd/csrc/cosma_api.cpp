@@ -80,9 +80,10 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // Heuristic tile / split choice, used when neither the caller nor the tuned
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
-//  - FP32 outputs in the range of its fitted cost model (fewer 256x128 tiles
-//    than twice the CUs, both sides >= 256, 512 <= K <= 32768): fp32_medium;
-//    the FP32 rules below cover the rest;
+//  - FP32 / FP64 outputs in the range of their fitted cost models (fewer
+//    256x128 / 128x128 tiles than two / four times the CUs, both sides >=
+//    256, 512 <= K <= 32768): fp_medium; the FP32 / FP64 rules below cover
+//    the rest;
 //  - BF16x3 / BF16x4 large outputs (the 256x256 tiles give every CU one):
 //    the 256x256 tile unsplit, over pre-split operands (planar_cfg) when its
 //    rounds of workgroups (ceil(tiles / CUs)) times 5504 reach M + N: the
@@ -227,47 +228,70 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   return Choice{1, clamp_split(cdiv(3 * (int64_t)cus, t1), k, 128)};
 }
 
-// FP32 outputs whose 256x128 tiles number fewer than twice the CUs, both
-// sides at least 256, 512 <= K <= 32768: the (tile, split) of least modelled
-// time.  The model is fitted (least squares on relative error) to full tuner
-// tables of the three BF16 shape sets run in FP32 on one MI355X (197 shapes,
-// cache flushed, every tile configuration at K splits 1, 2, 4, ...,
-// floor(CUs / tiles) and floor(2 CUs / tiles)); its time for a tile of t
-// tiles split s ways over slots workgroups per CU is, in microseconds,
+// FP32 and FP64 medium outputs: the (tile, split) of least modelled time.
+// Each mode's model is fitted (least squares on relative error) to full
+// tuner tables of the three BF16 shape sets run in that mode on one MI355X
+// (197 shapes, cache flushed, every tile configuration at K splits 1, 2, 4,
+// ..., floor(CUs / tiles) and floor(2 CUs / tiles)).  Its time for a tile
+// configuration of t tiles split s ways, slots workgroups per CU, is, in
+// microseconds,
 //   round_k * R * ks + load_k * L * ks + round * R + base
 //     + edge_k * R * ks (M, N or K not a multiple of the tile: guarded loads)
-//     + 205.432 us/GB * (2s + 1) * M * N * 4 bytes (split-K partials, s > 1),
+//     + partial_us_per_gb * (2s + 1) * M * N * element bytes (s > 1),
 // with R = ceil(t*s / (slots*CUs)) rounds, L = ceil(t*s / CUs) workgroups on
-// the busiest CU and ks the K slice in thousands.  Its error is 4.4% RMS over
-// all candidates; its picks are 1.4% / 1.3% / 1.5% slower than the fastest
-// candidate in the geometric mean on the three sets (fitted on the first two
-// only, 1.8% on the third), where the rules they replace were 17-19% slower
-// (101 of 197 shapes beyond 10%, up to 2.2x: splits rounded up past one round
-// of workgroups, no split below K = 1024).
+// the busiest CU and ks the K slice in thousands, rounded up to the tile's
+// K chunk.  Splits tried: 1, 2, 4, ..., 64 and those filling one or two
+// workgroups per CU, slices at least 64 deep.
+//  - FP32 (fewer 256x128 tiles than twice the CUs, both sides >= 256, 512 <=
+//    K <= 32768): 4.4% RMS error over all candidates; picks 1.4% / 1.3% /
+//    1.5% slower than the fastest candidate in the geometric mean on the
+//    three sets (fitted on the first two only: 1.8% on the third), where the
+//    rules they replace were 17-19% slower (101 of 197 shapes beyond 10%, up
+//    to 2.2x: splits rounded up past one round of workgroups, no split below
+//    K = 1024).
+//  - FP64 (fewer 128x128 tiles than four times the CUs, same sides and K):
+//    3.6% RMS error; picks 0.7% / 0.6% / 1.9% slower (fitted on the first set
+//    only: 0.8% and 2.2% on the others), where the rules they replace were
+//    25-27% slower (158 of 197 shapes beyond 10%, up to 3.3x).
 struct CostRow {
-  int cfg, bm, bn, slots;
+  int cfg, bm, bn, bk, slots;
   double round_k, load_k, round, base, edge_k;
 };
 constexpr CostRow kFp32Cost[] = {
-    {7, 256, 128, 1, 59.8401, 59.8401, 3.4542, 10.3971, 8.04107},
-    {8, 128, 256, 1, 64.4791, 64.4791, 2.44158, 10.642, 3.68721},
-    {9, 128, 128, 1, 33.1662, 33.1662, 1.00551, 10.5337, 6.97167},
-    {2, 128, 64, 3, 14.1125, 27.9115, 0.276966, 12.3699, 12.537},
-    {1, 64, 64, 4, 21.8173, 11.2695, 0.983808, 9.25921, 15.1252},
+    {7, 256, 128, 32, 1, 59.8401, 59.8401, 3.4542, 10.3971, 8.04107},
+    {8, 128, 256, 32, 1, 64.4791, 64.4791, 2.44158, 10.642, 3.68721},
+    {9, 128, 128, 32, 1, 33.1662, 33.1662, 1.00551, 10.5337, 6.97167},
+    {2, 128, 64, 32, 3, 14.1125, 27.9115, 0.276966, 12.3699, 12.537},
+    {1, 64, 64, 32, 4, 21.8173, 11.2695, 0.983808, 9.25921, 15.1252},
 };
-constexpr double kPartialUsPerGB = 205.432;
-constexpr bool fp32_model_applies(int m, int n, int k, int cus) {
-  return tiles_of(0, 7, m, n) < 2 * cus && m >= 256 && n >= 256 && k >= 512 && k <= 32768;
+constexpr CostRow kFp64Cost[] = {
+    {7, 128, 128, 16, 2, 18.1433, 107.886, 5.54048, 10.2357, 14.2741},
+    {3, 128, 128, 16, 2, 19.1521, 107.514, 4.51546, 10.4886, 13.7692},
+    {8, 128, 64, 16, 2, 14.5133, 55.2041, 1.9413, 11.6353, 13.3135},
+    {9, 64, 128, 16, 2, 20.8684, 52.8095, 1.16722, 12.5772, 10.3401},
+    {10, 64, 64, 16, 4, 26.7939, 25.5106, 1.36409, 10.3763, 8.42613},
+    {6, 256, 128, 16, 1, 118.868, 118.868, 8.31324, 11.4856, 24.4202},
+    {5, 128, 128, 32, 1, 62.3583, 62.3583, 4.51367, 10.982, 15.7249},
+};
+constexpr bool model_applies(int mode, int m, int n, int k, int cus) {
+  if (m < 256 || n < 256 || k < 512 || k > 32768) return false;
+  if (mode == 0) return tiles_of(0, 7, m, n) < 2 * cus;
+  if (mode == 1) return tiles_of(1, 7, m, n) < 4 * cus;
+  return false;
 }
-constexpr Choice fp32_medium(int m, int n, int k, int cus) {
-  Choice best{7, 1};
+template <int N_ROWS>
+constexpr Choice model_pick(const CostRow (&rows)[N_ROWS], double partial_us_per_gb, int elem_bytes, int m, int n,
+                            int k, int cus) {
+  // No fused multiply-adds: the run-time evaluation rounds exactly as the
+  // compile-time one of the fixed-shape entry points.
+#pragma clang fp contract(off)
+  Choice best{rows[0].cfg, 1};
   double best_us = 0.0;
   bool have = false;
-  for (const CostRow& c : kFp32Cost) {
+  for (const CostRow& c : rows) {
     const int64_t t = (int64_t)cdiv(m, c.bm) * cdiv(n, c.bn);
-    const bool edge = m % c.bm || n % c.bn || k % 32;
-    // Splits 1, 2, 4, ..., 64 and those filling one or two workgroups per CU,
-    // in ascending order (ties keep the first, smallest split).
+    const bool edge = m % c.bm || n % c.bn || k % c.bk;
+    // Candidate splits in ascending order (ties keep the first, smallest).
     int cand[9] = {1, 2, 4, 8, 16, 32, 64, (int)(cus / t > 1 ? cus / t : 1), (int)(2 * cus / t > 1 ? 2 * cus / t : 1)};
     for (int i = 1; i < 9; ++i)
       for (int j = i; j > 0 && cand[j] < cand[j - 1]; --j) {
@@ -282,9 +306,9 @@ constexpr Choice fp32_medium(int m, int n, int k, int cus) {
       const int64_t wg = t * s;
       const double R = (double)cdiv(wg, (int64_t)c.slots * cus);
       const double L = (double)cdiv(wg, cus);
-      const double ks = (double)((int64_t)cdiv(cdiv(k, s), 32) * 32) / 1000.0;
+      const double ks = (double)((int64_t)cdiv(cdiv(k, s), c.bk) * c.bk) / 1000.0;
       double us = c.round_k * R * ks + c.load_k * L * ks + c.round * R + c.base + (edge ? c.edge_k * R * ks : 0.0);
-      if (s > 1) us += kPartialUsPerGB * (2.0 * s + 1.0) * (double)m * (double)n * 4.0 / 1e9;
+      if (s > 1) us += partial_us_per_gb * (2.0 * s + 1.0) * (double)m * (double)n * elem_bytes / 1e9;
       if (!have || us < best_us) {
         best_us = us;
         best = Choice{c.cfg, s};
@@ -293,6 +317,9 @@ constexpr Choice fp32_medium(int m, int n, int k, int cus) {
     }
   }
   return best;
+}
+constexpr Choice fp_medium(int mode, int m, int n, int k, int cus) {
+  return mode == 0 ? model_pick(kFp32Cost, 205.432, 4, m, n, k, cus) : model_pick(kFp64Cost, 137.163, 8, m, n, k, cus);
 }
 
 // ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A
@@ -304,7 +331,7 @@ constexpr Choice fp32_medium(int m, int n, int k, int cus) {
 constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
   // An empty output launches nothing; a non-positive CU count is not a device.
   if (m <= 0 || n <= 0 || cus <= 0) return Choice{medium_cfg(mode), 1};
-  if (mode == 0 && fp32_model_applies(m, n, k, cus)) return fp32_medium(m, n, k, cus);
+  if (model_applies(mode, m, n, k, cus)) return fp_medium(mode, m, n, k, cus);
   int cfg = large_cfg(mode);
   const int lt = tiles_of(mode, cfg, m, n);
   if (split_mode(mode)) {

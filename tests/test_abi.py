@@ -79,17 +79,17 @@ def test_heuristic_round_rules():
     import cucosma_amd
 
     h = cucosma_amd.heuristic
-    # FP64 medium outputs: smallest split whose last round (two 128x64 or four
-    # 64x64 workgroups per CU) is at least three quarters full.
-    assert h(1, 1024, 2048, 2048) == (2, 2)   # 256 tiles: split 3 would leave 1/2 of a second round
-    assert h(1, 2048, 1024, 1024) == (2, 2)
-    assert h(1, 1792, 1792, 1792) == (2, 1)   # 392 tiles: 0.77 of one round
-    assert h(1, 1536, 1536, 1536) == (2, 3)
-    assert h(1, 1024, 1024, 1024) == (1, 3)
+    # FP64 medium outputs (fewer 128x128 tiles than four times the CUs): the
+    # fitted cost model; each pick was the tuner's fastest or within 1%.
+    assert h(1, 1280, 1280, 8192) == (3, 5)
+    assert h(1, 1536, 1536, 16384) == (6, 7)
+    assert h(1, 640, 640, 4096) == (8, 10)
+    assert h(1, 384, 384, 512) == (10, 7)
+    assert h(1, 3072, 3072, 2048) == (10, 1)
+    assert h(1, 1024, 1024, 1024) == (8, 2)
     # FP64 large outputs: split in two when the last round of 512 is at most 1/4 full.
-    assert h(1, 3072, 3072, 3072) == (7, 2)   # 576 tiles
-    assert h(1, 4608, 2304, 2048) == (7, 1)   # 648 tiles: 136 of 512
     assert h(1, 4096, 4096, 4096) == (7, 1)   # whole rounds
+    assert h(1, 32768, 1024, 1024) == (7, 1)  # 2048 tiles: whole rounds
     # BF16: pre-split tile when ceil(tiles / 256) * 5504 >= M + N.
     for mode in (2, 3):
         assert h(mode, 4608, 4608, 4096) == (8, 1)
@@ -299,16 +299,16 @@ def test_heuristic_follows_the_cu_count():
     # Nominal topology is the default without a device.
     assert h(2, 8192, 8192, 8192, cus=256) == h(2, 8192, 8192, 8192)
     assert h(2, 8192, 8192, 8192, cus=128) == (8, 1)
-    # FP64 large outputs: split in two when the last round of 2 * cus is at most a quarter full.
-    assert h(1, 3072, 3072, 3072, cus=256) == (7, 2)   # 576 tiles: last round 64 of 512
-    assert h(1, 3072, 3072, 3072, cus=128) == (7, 2)   # last round 64 of 256
-    assert h(1, 2304, 2304, 2048, cus=128) == (7, 1)   # 324 tiles: last round 68 of 256
-    # 400 128x128 tiles are "large" (two rounds of workgroups) only on 128 CUs.
-    assert h(1, 2560, 2560, 2048, cus=256)[0] == 3
-    assert h(1, 2560, 2560, 2048, cus=128) == (7, 1)
-    # The medium FP64 rule counts slots of the given CUs (1024^3: 256 64x64 tiles).
-    assert h(1, 1024, 1024, 1024, cus=256) == (1, 3)
-    assert h(1, 1024, 1024, 1024, cus=64)[1] == 1
+    # FP64 large outputs (at least four 128x128 tiles per CU): split in two
+    # when the last round of 2 * cus is at most a quarter full.
+    assert h(1, 3072, 3072, 3072, cus=128) == (7, 2)   # 576 tiles: last round 64 of 256
+    assert h(1, 4608, 2304, 2048, cus=128) == (7, 1)   # 648 tiles: last round 136 of 256
+    # The FP64 cost model counts rounds and per-CU loads of the given CUs: the
+    # same shape gets another tile or split on fewer CUs.
+    assert h(1, 2560, 2560, 2048, cus=256) == (10, 1)
+    assert h(1, 2560, 2560, 2048, cus=128) == (3, 4)
+    assert h(1, 1024, 1024, 1024, cus=256) == (8, 2)
+    assert h(1, 1024, 1024, 1024, cus=64) == (3, 2)
     lib = cucosma_amd.lib
     out = (C.c_int(), C.c_int())
     assert lib.cosma_heuristic_topo(0, 64, 64, 64, 0, 0, C.byref(out[0]), C.byref(out[1])) == -1

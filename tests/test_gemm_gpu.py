@@ -202,10 +202,10 @@ def test_fp64_vs_long_double_with_outliers(cc, dev):
 
 
 def test_fp64_large_output_split_in_two(cc, dev):
-    """FP64 3072^3: 576 tiles of 128x128 leave the second round of 512
+    """FP64 4352x4096x2048: 1088 tiles of 128x128 leave the third round of 512
     workgroups 1/8 full, so the heuristic splits K in two; the result stays
     within 2*K*u*|A||B| (u = 2^-53) of torch's FP64 product."""
-    M = N = K = 3072
+    M, N, K = 4352, 4096, 2048
     assert cc.heuristic(1, M, N, K) == (7, 2)
     g = torch.Generator(device="cpu").manual_seed(21)
     A = torch.randn(M, K, generator=g, dtype=torch.float64).to(dev)
