@@ -235,43 +235,46 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
 // ..., floor(CUs / tiles) and floor(2 CUs / tiles)).  Its time for a tile
 // configuration of t tiles split s ways, slots workgroups per CU, is, in
 // microseconds,
-//   round_k * R * ks + load_k * L * ks + round * R + base
-//     + edge_k * R * ks (M, N or K not a multiple of the tile: guarded loads)
+//   round_k * R * ks + load_k * L * ks + round * R + base + wg * L
+//     + edge_k * R * ks (M, N or K not a multiple of the tile)
 //     + partial_us_per_gb * (2s + 1) * M * N * element bytes (s > 1),
 // with R = ceil(t*s / (slots*CUs)) rounds, L = ceil(t*s / CUs) workgroups on
 // the busiest CU and ks the K slice in thousands, rounded up to the tile's
 // K chunk.  Splits tried: 1, 2, 4, ..., 64 and those filling one or two
-// workgroups per CU, slices at least 64 deep.
+// workgroups per CU, slices at least 64 deep.  The edge term mostly prices
+// leading dimensions that are not whole 128-byte lines (1024x1000x4096: 64x64
+// and 128x64 tiles 16-20% slower than 1024^3, with or without the loaders'
+// guards).
 //  - FP32 (fewer 256x128 tiles than twice the CUs, both sides >= 256, 512 <=
-//    K <= 32768): 4.4% RMS error over all candidates; picks 1.4% / 1.3% /
-//    1.5% slower than the fastest candidate in the geometric mean on the
-//    three sets (fitted on the first two only: 1.8% on the third), where the
+//    K <= 32768): 4.3% RMS error over all candidates; picks 0.7% / 0.9% /
+//    0.6% slower than the fastest candidate in the geometric mean on the
+//    three sets (fitted on the first two only: 1.1% on the third), where the
 //    rules they replace were 17-19% slower (101 of 197 shapes beyond 10%, up
 //    to 2.2x: splits rounded up past one round of workgroups, no split below
 //    K = 1024).
 //  - FP64 (fewer 128x128 tiles than four times the CUs, same sides and K):
-//    3.6% RMS error; picks 0.7% / 0.6% / 1.9% slower (fitted on the first set
-//    only: 0.8% and 2.2% on the others), where the rules they replace were
-//    25-27% slower (158 of 197 shapes beyond 10%, up to 3.3x).
+//    3.6% RMS error; picks 0.7% / 0.7% / 1.4% slower (fitted on the first two
+//    sets only: 1.5% on the third), where the rules they replace were 25-27%
+//    slower (158 of 197 shapes beyond 10%, up to 3.3x).
 struct CostRow {
   int cfg, bm, bn, bk, slots;
-  double round_k, load_k, round, base, edge_k;
+  double round_k, load_k, round, base, edge_k, wg;
 };
 constexpr CostRow kFp32Cost[] = {
-    {7, 256, 128, 32, 1, 59.8401, 59.8401, 3.4542, 10.3971, 8.04107},
-    {8, 128, 256, 32, 1, 64.4791, 64.4791, 2.44158, 10.642, 3.68721},
-    {9, 128, 128, 32, 1, 33.1662, 33.1662, 1.00551, 10.5337, 6.97167},
-    {2, 128, 64, 32, 3, 14.1125, 27.9115, 0.276966, 12.3699, 12.537},
-    {1, 64, 64, 32, 4, 21.8173, 11.2695, 0.983808, 9.25921, 15.1252},
+    {7, 256, 128, 32, 1, 59.8721, 59.8721, 2.22289, 9.91833, 7.87432, 2.22289},
+    {8, 128, 256, 32, 1, 64.5165, 64.5165, 1.71571, 10.1618, 3.51048, 1.71571},
+    {9, 128, 128, 32, 1, 33.1659, 33.1659, 0.755999, 10.3085, 6.89528, 0.755999},
+    {2, 128, 64, 32, 3, 14.8936, 27.4337, -2.63133, 13.7061, 12.4718, 1.25521},
+    {1, 64, 64, 32, 4, 22.7421, 10.8407, -2.2406, 10.7836, 15.1856, 0.954249},
 };
 constexpr CostRow kFp64Cost[] = {
-    {7, 128, 128, 16, 2, 18.1433, 107.886, 5.54048, 10.2357, 14.2741},
-    {3, 128, 128, 16, 2, 19.1521, 107.514, 4.51546, 10.4886, 13.7692},
-    {8, 128, 64, 16, 2, 14.5133, 55.2041, 1.9413, 11.6353, 13.3135},
-    {9, 64, 128, 16, 2, 20.8684, 52.8095, 1.16722, 12.5772, 10.3401},
-    {10, 64, 64, 16, 4, 26.7939, 25.5106, 1.36409, 10.3763, 8.42613},
-    {6, 256, 128, 16, 1, 118.868, 118.868, 8.31324, 11.4856, 24.4202},
-    {5, 128, 128, 32, 1, 62.3583, 62.3583, 4.51367, 10.982, 15.7249},
+    {7, 128, 128, 16, 2, 17.162, 108.609, 6.88039, 9.63477, 14.1804, -0.161412},
+    {3, 128, 128, 16, 2, 17.5772, 108.666, 6.79159, 9.59746, 13.6972, -0.641176},
+    {8, 128, 64, 16, 2, 15.8368, 54.3138, -0.81831, 12.3499, 13.2295, 1.67745},
+    {9, 64, 128, 16, 2, 22.6093, 51.6528, -2.34361, 13.4991, 10.2409, 2.05803},
+    {10, 64, 64, 16, 4, 28.2026, 24.8464, -3.28806, 12.6808, 8.52273, 1.31925},
+    {6, 256, 128, 16, 1, 118.918, 118.918, 4.67968, 11.0669, 24.1654, 4.67968},
+    {5, 128, 128, 32, 1, 62.3576, 62.3576, 2.53373, 10.7562, 15.6285, 2.53373},
 };
 constexpr bool model_applies(int mode, int m, int n, int k, int cus) {
   if (m < 256 || n < 256 || k < 512 || k > 32768) return false;
@@ -307,7 +310,8 @@ constexpr Choice model_pick(const CostRow (&rows)[N_ROWS], double partial_us_per
       const double R = (double)cdiv(wg, (int64_t)c.slots * cus);
       const double L = (double)cdiv(wg, cus);
       const double ks = (double)((int64_t)cdiv(cdiv(k, s), c.bk) * c.bk) / 1000.0;
-      double us = c.round_k * R * ks + c.load_k * L * ks + c.round * R + c.base + (edge ? c.edge_k * R * ks : 0.0);
+      double us = c.round_k * R * ks + c.load_k * L * ks + c.round * R + c.base + c.wg * L +
+                  (edge ? c.edge_k * R * ks : 0.0);
       if (s > 1) us += partial_us_per_gb * (2.0 * s + 1.0) * (double)m * (double)n * elem_bytes / 1e9;
       if (!have || us < best_us) {
         best_us = us;
@@ -319,7 +323,7 @@ constexpr Choice model_pick(const CostRow (&rows)[N_ROWS], double partial_us_per
   return best;
 }
 constexpr Choice fp_medium(int mode, int m, int n, int k, int cus) {
-  return mode == 0 ? model_pick(kFp32Cost, 205.432, 4, m, n, k, cus) : model_pick(kFp64Cost, 137.163, 8, m, n, k, cus);
+  return mode == 0 ? model_pick(kFp32Cost, 190.532, 4, m, n, k, cus) : model_pick(kFp64Cost, 128.974, 8, m, n, k, cus);
 }
 
 // ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A
