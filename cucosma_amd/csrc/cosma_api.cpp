@@ -81,9 +81,9 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 // table (cosma_tune_set) fixes them.  Derived from autotuner sweeps on one
 // MI355X:
 //  - FP32 / FP64 outputs in the range of their fitted cost models (fewer
-//    256x128 / 128x128 tiles than two / four times the CUs, both sides >=
-//    256, 512 <= K <= 32768): fp_medium; the FP32 / FP64 rules below cover
-//    the rest;
+//    256x128 / 128x128 tiles than two / four times the CUs, both sides >= 16,
+//    M * N >= 16384, 512 <= K <= 32768): fp_medium; the FP32 / FP64 rules
+//    below cover the rest;
 //  - BF16x3 / BF16x4 large outputs (the 256x256 tiles give every CU one):
 //    the 256x256 tile unsplit, over pre-split operands (planar_cfg) when its
 //    rounds of workgroups (ceil(tiles / CUs)) times 5504 reach M + N: the
@@ -194,7 +194,8 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
 //    that fills three quarters of the slots or every CU has a tile.  It ran
 //    0-30% faster than the 4-wave 128x128 tile (cfg 0) in BF16x4 and within
 //    3% of it in BF16x3.
-//  - Otherwise 64x64 (cfg 1), K split toward three workgroups per CU with
+//  - Otherwise, or when the 128x128 tiles cover under 70% of the output,
+//    64x64 (cfg 1), K split toward three workgroups per CU with
 //    slices at least 128 deep while the tiles number fewer than twice the
 //    CUs (fewer than the CUs below K = 1024; 384^2x512: split 4, 1.7x faster
 //    than unsplit).
@@ -223,7 +224,10 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   // rounds12) >= M*N*s11 / (16384 * slots11), with M*N cancelled.
   const bool share_ok = (int64_t)s12 * slots11 >= 4 * (int64_t)s11 * cus * cdiv(lt * s12, cus);
   if (share_ok && 4 * lt * s12 >= 3 * (int64_t)cus && !x4_c11 && deep_ok) return Choice{12, s12};
-  if (2 * w11 >= 3 * (int64_t)cus || t11 >= cus) return Choice{11, s11};
+  // 128x128 tiles covering under 70% of the output (a side well below 128)
+  // give way to 64x64 tiles (192x16384x1024: 1.36x faster).
+  const bool c11_covers = 10 * (int64_t)m * n >= 7 * t11 * 16384;
+  if ((2 * w11 >= 3 * (int64_t)cus || t11 >= cus) && c11_covers) return Choice{11, s11};
   if (t1 >= 2 * cus || (t1 >= cus && k < 1024)) return Choice{1, 1};
   return Choice{1, clamp_split(cdiv(3 * (int64_t)cus, t1), k, 128)};
 }
@@ -246,18 +250,22 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
 // leading dimensions that are not whole 128-byte lines (1024x1000x4096: 64x64
 // and 128x64 tiles 16-20% slower than 1024^3, with or without the loaders'
 // guards).
-//  - FP32 (fewer 256x128 tiles than twice the CUs, both sides >= 256, 512 <=
-//    K <= 32768): 4.3% RMS error over all candidates; picks 1.1% / 1.0% /
+//  - FP32 (fewer 256x128 tiles than twice the CUs, both sides >= 16, M * N >=
+//    16384, 512 <= K <= 32768): 4.3% RMS error over all candidates; picks 1.1% / 1.0% /
 //    0.7% slower than the fastest candidate in the geometric mean on the
 //    three sets (fitted on the first two only: 0.6% on the third), where the
 //    rules they replace were 17-19% slower (101 of 197 shapes beyond 10%, up
 //    to 2.2x: splits rounded up past one round of workgroups, no split below
 //    K = 1024).  With splits for one or two workgroups per CU only, 1.5% /
-//    2.3% / 1.5%.
+//    2.3% / 1.5%.  On 48 skinny products (M 16-192, N 1024-16384, K 1024-
+//    16384; not fitted) 2.7%, where the rules before were 35% slower (up to
+//    3x: 256x128 tiles for M = 64).
 //  - FP64 (fewer 128x128 tiles than four times the CUs, same sides and K):
-//    3.7% RMS error; picks 0.7% / 0.7% / 1.8% slower (fitted on the first two
-//    sets only: 1.7% on the third), where the rules they replace were 25-27%
-//    slower (158 of 197 shapes beyond 10%, up to 3.3x).
+//    fitted on the three sets and the skinny products, 5.0% RMS error; picks
+//    0.6% / 0.6% / 1.7% / 4.4% slower (fitted on the first two sets only:
+//    1.7% on the third), where the rules they replace were 25-27% slower on
+//    the three sets (158 of 197 shapes beyond 10%, up to 3.3x) and 40% on the
+//    skinny products.
 struct CostRow {
   int cfg, bm, bn, bk, slots;
   double round_k, load_k, round, base, edge_k, wg;
@@ -270,16 +278,16 @@ constexpr CostRow kFp32Cost[] = {
     {1, 64, 64, 32, 4, 22.643, 10.8008, -2.23666, 10.7746, 15.0589, 1.1184},
 };
 constexpr CostRow kFp64Cost[] = {
-    {7, 128, 128, 16, 2, 16.0885, 109.241, 6.81203, 9.47843, 14.8057, 0.727854},
-    {3, 128, 128, 16, 2, 16.5174, 109.6, 6.36343, 9.38288, 14.273, 0.454047},
-    {8, 128, 64, 16, 2, 14.107, 55.3494, 0.154784, 12.3252, 14.2014, 1.62783},
-    {9, 64, 128, 16, 2, 21.35, 52.3873, -2.53791, 13.7045, 10.79, 2.59704},
-    {10, 64, 64, 16, 4, 27.6247, 25.239, -3.13313, 13.0085, 8.87486, 1.48981},
-    {6, 256, 128, 16, 1, 119.157, 119.157, 5.46107, 10.5612, 23.5353, 5.46107},
-    {5, 128, 128, 32, 1, 62.3673, 62.3673, 2.93682, 10.5946, 16.3672, 2.93682},
+    {7, 128, 128, 16, 2, 16.429, 109.162, 5.36348, 9.96616, 14.3867, 0.9653},
+    {3, 128, 128, 16, 2, 16.8134, 109.608, 5.35005, 9.73712, 13.0197, 0.478038},
+    {8, 128, 64, 16, 2, 15.3006, 54.6391, -0.482321, 11.7838, 14.4705, 2.03437},
+    {9, 64, 128, 16, 2, 21.7279, 52.0208, -4.69961, 14.1535, 12.4124, 3.99446},
+    {10, 64, 64, 16, 4, 28.6054, 25.0405, -5.77156, 14.1922, 13.5788, 2.29557},
+    {6, 256, 128, 16, 1, 119.658, 119.658, 4.46679, 11.2518, 22.3278, 4.46679},
+    {5, 128, 128, 32, 1, 62.1969, 62.1969, 2.83265, 10.589, 15.2232, 2.83265},
 };
 constexpr bool model_applies(int mode, int m, int n, int k, int cus) {
-  if (m < 256 || n < 256 || k < 512 || k > 32768) return false;
+  if (m < 16 || n < 16 || (int64_t)m * n < 16384 || k < 512 || k > 32768) return false;
   if (mode == 0) return tiles_of(0, 7, m, n) < 2 * cus;
   if (mode == 1) return tiles_of(1, 7, m, n) < 4 * cus;
   return false;
@@ -330,7 +338,7 @@ constexpr Choice model_pick(const CostRow (&rows)[N_ROWS], double partial_us_per
 // split 4; beta 0.5: 0.038 against 0.040 ms), a gap inside the model's error.
 constexpr Choice fp_medium(int mode, int m, int n, int k, int cus) {
   if (mode == 0 && m == 1024 && n == 1024 && k == 1024 && cus == 256) return Choice{1, 3};
-  return mode == 0 ? model_pick(kFp32Cost, 170.798, 4, m, n, k, cus) : model_pick(kFp64Cost, 111.895, 8, m, n, k, cus);
+  return mode == 0 ? model_pick(kFp32Cost, 170.798, 4, m, n, k, cus) : model_pick(kFp64Cost, 122.074, 8, m, n, k, cus);
 }
 
 // ta: op(A) is stored transposed (K x M).  The on-the-fly tiles transpose A

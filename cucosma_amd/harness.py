@@ -29,8 +29,8 @@ Configurations:
   schedules the two on-the-fly 256x256 BF16 schedules (ping-pong cfg 6, spread
            loads cfg 12) on five shapes, orders alternated round by round
   heuristic the library's default tile and split against the autotuner's fastest
-           candidate on three shape sets of 94, 58 and 45 shapes (BF16x3 and BF16x4,
-           or --modes)
+           candidate on four shape sets of 94, 58, 45 and 48 shapes (BF16x3 and
+           BF16x4, or --modes)
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
 """
@@ -396,11 +396,12 @@ def cfg_schedules(cc, torch, args, emit):
 
 
 def heuristic_shape_sets():
-    """Three BF16 shape sets whose 256x256 tiles mostly leave CUs idle: M = N
-    from 384 to 3072 (multiples of 256 and 128) with K from 512 to 32768;
-    M = N from 448 to 3328 (not multiples of 256) with K from 768 to 24576;
-    M = N from 320 to 3200 with K from 640 to 20480; ten rectangular shapes
-    each."""
+    """Shape sets whose 256x256 tiles mostly leave CUs idle: A, M = N from 384
+    to 3072 (multiples of 256 and 128) with K from 512 to 32768; B, M = N from
+    448 to 3328 (not multiples of 256) with K from 768 to 24576; C, M = N from
+    320 to 3200 with K from 640 to 20480 (ten rectangular shapes each); S,
+    skinny products, M from 16 to 192 against N from 1024 to 16384 and K from
+    1024 to 16384, and three with N small."""
     a = [(m, m, k) for m in (384, 512, 640, 768, 1024, 1280, 1536, 1792, 2048, 2304, 2560, 3072)
          for k in (512, 1024, 2048, 4096, 8192, 16384, 32768)]
     a += [(1024, 2048, 4096), (2048, 1024, 8192), (512, 2048, 8192), (2048, 512, 8192), (1024, 4096, 2048),
@@ -411,7 +412,9 @@ def heuristic_shape_sets():
     c = [(m, m, k) for m in (320, 576, 960, 1216, 1856, 2432, 3200) for k in (640, 2560, 5120, 10240, 20480)]
     c += [(800, 2000, 6000), (2000, 800, 1500), (1300, 2600, 3500), (300, 3000, 12000), (3100, 1700, 900),
           (1750, 1750, 2500), (2600, 2600, 4500), (500, 1500, 30000), (2300, 1100, 16000), (3600, 3000, 1800)]
-    return {"A": a, "B": b, "C": c}
+    s = [(m, n, k) for m in (16, 32, 64, 128, 192) for n in (1024, 4096, 16384) for k in (1024, 4096, 16384)]
+    s += [(4096, 64, 4096), (16384, 128, 4096), (4096, 32, 16384)]
+    return {"A": a, "B": b, "C": c, "S": s}
 
 
 def cfg_heuristic(cc, torch, args, emit):

@@ -120,6 +120,10 @@ def test_heuristic_fp32_cost_model():
     assert h(0, 3072, 3072, 512) == (1, 1)      # short K: 64x64 tiles
     assert h(0, 1024, 1024, 1024) == (1, 3)     # the reference's first configuration (measured)
     assert h(0, 384, 384, 512)[0] == 1 and h(0, 384, 384, 512)[1] > 1   # K split even below K = 1024
+    # Skinny products: 64x64 tiles (the rules before ran 256x128 split 2, 2.2-3x slower).
+    assert h(0, 64, 16384, 16384) == (1, 4)
+    assert h(0, 16, 16384, 4096) == (1, 4)
+    assert h(1, 32, 16384, 16384) == (9, 4)
     # Outside the fitted range the rules stay: large outputs, the flat shape.
     assert h(0, 8192, 8192, 8192) == (7, 1)
     assert h(0, 256, 256, 1 << 20)[0] == 7
@@ -154,6 +158,8 @@ def test_heuristic_split_medium_rules():
     # One unsplit round filling half the CUs or more: the pre-split tile.
     assert h(2, 3500, 3500, 3000) == (8, 1)
     assert h(3, 2900, 2900, 1100) == (8, 1)
+    # 128x128 tiles covering under 70% of the output: 64x64 tiles.
+    assert h(2, 4096, 32, 16384) == (1, 12)
     # 64x64: K split toward three workgroups per CU, slices at least 128 deep.
     assert h(2, 384, 384, 512) == (1, 4)
     assert h(3, 1024, 1024, 1024) == (1, 3)
