@@ -94,7 +94,9 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    rounds, 6-8% faster in both split modes; 4096^3, 2048x8192x2048 and
 //    4096x8192x4096 tie or lose and run on the fly; 5632^2x2048 lost 1% and
 //    is just outside).  5504 >= 2^24 / 3072, so every output with
-//    M*N/(M+N) >= 3072 also qualifies (BF16x4: 6144).  The 256x256 tile beat
+//    M*N/(M+N) >= 3072 also qualifies (BF16x4: 6144).  Outputs with edge
+//    tiles (tile_edges) pre-split from K = 512 (BF16x3 with one round of
+//    tiles: from K = 2048).  The 256x256 tile beat
 //    every smaller tile from 2048^3 up.  Smaller outputs: split_medium;
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    in 16 waves of 64x32 with the next stage held in registers (four waves
@@ -155,6 +157,18 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   if (split > max_split) split = max_split;
   return split < 1 ? 1 : split;
 }
+// An output side that is not a multiple of 256 (BF16x4: neither of 256 nor of
+// 128) leaves partly filled 256x256 tiles, which cost the on-the-fly tiles
+// more than the pre-split one, whose plane rows hold K rounded up to 32 and so
+// start on cache lines, where cfg 12 reads the FP32 operands in place, rows
+// straddling cache lines when the leading dimension (here M or N) is not a
+// multiple of 32.  BF16x4 outputs whose sides are multiples of 128 run
+// fastest on the 128x128 and 256x256 on-the-fly tiles (3200^2).
+constexpr bool tile_edges(int mode, int m, int n) {
+  const bool e256 = m % 256 != 0 || n % 256 != 0;
+  const bool e128 = m % 128 != 0 || n % 128 != 0;
+  return e256 && (mode != 3 || e128);
+}
 // BF16x3 / BF16x4 outputs whose 256x256 tiles do not give every CU a
 // workgroup.  Rules read off tuner tables of three shape sets on one MI355X
 // (cache flushed; every tile configuration at K splits 1, 2, 4, ...,
@@ -182,6 +196,16 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
 //    tile area, times the filled share of its rounds) is at least the 128x128
 //    tiles' (1152^2 to 1664^2: 81-86% against 95-99%, 128x128 tiles 3-26%
 //    faster; 1856^2: 82% against 82%, this tile 6-27% faster in BF16x3).
+//  - Before those two rules, 256x256 tiles filling at least half the CUs and
+//    covering the output at least 88%, K >= 512: with edge tiles
+//    (tile_edges) the pre-split tile at the split above (2900^2 x 3072 to
+//    8192: split 3, 0-2% from the fastest candidate against 7-13% for cfg
+//    12; 3500^2 and 3700^2 BF16x4, K <= 2048: 5-20% faster than 128x128);
+//    BF16x3 with sides multiples of 256 and K >= 1536 the spread-load tile
+//    at that split (3072^2 and 3328^2 x 1536 to 2048: 6-8% faster than
+//    pre-split).  A band table of M = N from 2900 to 3900 x K from 512 to
+//    8192 went from 1.8% / 4.5% slower than the fastest candidate in the
+//    geometric mean (BF16x3 / BF16x4) to 0.6% / 0.5%.
 //  - One round of unsplit 256x256 workgroups filling at least half the CUs,
 //    tiles covering the output at least 88%, K >= 512 (BF16x4: 1024): the
 //    pre-split tile (3500^2x3000 16-17% faster than cfg 12; 3072^2x1024
@@ -216,6 +240,10 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   int s12 = cost(sa) <= cost(sb) ? sa : sb;
   if (s12 == sb && s12 > 1 && !(k >= 2048 && k / s12 >= depth)) s12 = sa;
   const bool cover_ok = 25 * (int64_t)m * n >= 22 * lt * 65536;
+  if (2 * lt >= cus && k >= 512 && cover_ok) {
+    if (tile_edges(mode, m, n)) return Choice{planar_cfg(mode), s12};
+    if (!x4 && m % 256 == 0 && n % 256 == 0 && k >= 1536) return Choice{12, s12};
+  }
   const bool x4_c11 = x4 && 2 * t11 >= 3 * (int64_t)cus && k <= 2048 && 5 * w11 >= 3 * slots11;
   if (2 * lt >= cus && s12 == 1 && k >= (x4 ? 1024 : 512) && !x4_c11 && cover_ok)
     return Choice{planar_cfg(mode), 1};
@@ -367,6 +395,14 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
       // either way).
       const int64_t per_round = mode == 3 ? 6144 : 5504;
       if (k >= 512 && ((ta && k >= 4096) || (int64_t)cdiv(lt, cus) * per_round >= (int64_t)m + n))
+        return Choice{planar_cfg(mode), 1};
+      // Edge tiles (tile_edges): pre-split whenever K >= 512, except BF16x3
+      // with one round of tiles and K < 2048 (3900x7900, 2000x16000 and
+      // 1500x20000, two rounds, 0-2% from the fastest candidate against
+      // 3-17% on the fly; 3900^2 and 4000^2, one round: BF16x3 2-5% slower
+      // pre-split below K = 2048, 0-10% faster from there; BF16x4 0-8% faster
+      // from K = 512).
+      if (k >= 512 && tile_edges(mode, m, n) && (mode == 3 || k >= 2048 || lt > cus))
         return Choice{planar_cfg(mode), 1};
       return Choice{cfg, 1};
     }

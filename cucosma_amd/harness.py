@@ -29,8 +29,8 @@ Configurations:
   schedules the two on-the-fly 256x256 BF16 schedules (ping-pong cfg 6, spread
            loads cfg 12) on five shapes, orders alternated round by round
   heuristic the library's default tile and split against the autotuner's fastest
-           candidate on four shape sets of 94, 58, 45 and 48 shapes (BF16x3 and
-           BF16x4, or --modes)
+           candidate on five shape sets of 94, 58, 45, 48 and 45 shapes (BF16x3
+           and BF16x4, or --modes)
 
 Usage: python -m cucosma_amd.harness --config square [--runs 100] [--warmup 10]
 """
@@ -401,7 +401,9 @@ def heuristic_shape_sets():
     448 to 3328 (not multiples of 256) with K from 768 to 24576; C, M = N from
     320 to 3200 with K from 640 to 20480 (ten rectangular shapes each); S,
     skinny products, M from 16 to 192 against N from 1024 to 16384 and K from
-    1024 to 16384, and three with N small."""
+    1024 to 16384, and three with N small; E, 256x256 tiles filling half the
+    CUs or more, M = N from 2900 to 4000 (edge tiles but 3072 and 3328) with
+    K from 512 to 8192, 4096^2 and four long rectangles."""
     a = [(m, m, k) for m in (384, 512, 640, 768, 1024, 1280, 1536, 1792, 2048, 2304, 2560, 3072)
          for k in (512, 1024, 2048, 4096, 8192, 16384, 32768)]
     a += [(1024, 2048, 4096), (2048, 1024, 8192), (512, 2048, 8192), (2048, 512, 8192), (1024, 4096, 2048),
@@ -414,7 +416,10 @@ def heuristic_shape_sets():
           (1750, 1750, 2500), (2600, 2600, 4500), (500, 1500, 30000), (2300, 1100, 16000), (3600, 3000, 1800)]
     s = [(m, n, k) for m in (16, 32, 64, 128, 192) for n in (1024, 4096, 16384) for k in (1024, 4096, 16384)]
     s += [(4096, 64, 4096), (16384, 128, 4096), (4096, 32, 16384)]
-    return {"A": a, "B": b, "C": c, "S": s}
+    e = [(m, m, k) for m in (2900, 3072, 3200, 3328, 3500, 3700, 3900) for k in (512, 1024, 2048, 4096, 8192)]
+    e += [(m, n, k) for (m, n) in ((4000, 4000), (4096, 4096), (3900, 7900), (2000, 16000), (1500, 20000))
+          for k in (512, 2048)]
+    return {"A": a, "B": b, "C": c, "S": s, "E": e}
 
 
 def cfg_heuristic(cc, torch, args, emit):

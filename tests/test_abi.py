@@ -165,6 +165,33 @@ def test_heuristic_split_medium_rules():
     assert h(3, 1024, 1024, 1024) == (1, 3)
 
 
+def test_heuristic_edge_tiles_presplit():
+    """BF16 outputs with partly filled 256x256 tiles (a side not a multiple of
+    256; BF16x4: nor of 128) take the pre-split tile once those tiles fill half
+    the CUs; aligned outputs keep the on-the-fly tiles.  Every pick here was
+    the tuner's fastest configuration for the shape (cache flushed, one
+    MI355X; host code, nominal 256 CUs)."""
+    import cucosma_amd
+
+    h = cucosma_amd.heuristic
+    # One round of 144 tiles: the pre-split tile at the 256x256 split.
+    assert h(2, 2900, 2900, 4096) == (8, 3)
+    assert h(3, 2900, 2900, 4096) == (8, 3)
+    assert h(3, 3700, 3700, 1536) == (8, 1)
+    # Sides multiples of 128: BF16x4 keeps 128x128 tiles.
+    assert h(3, 3200, 3200, 1536) == (11, 1)
+    # Aligned, BF16x3 from K = 1536: the spread-load tile.
+    assert h(2, 3328, 3328, 1536) == (12, 1)
+    assert h(2, 3072, 3072, 4096) == (12, 3)
+    assert h(2, 3072, 3072, 1024) == (8, 1)
+    # Full rounds: pre-split with edges (BF16x3 with one round from K = 2048).
+    assert h(2, 3900, 7900, 1024) == (8, 1)
+    assert h(2, 3900, 3900, 1024) == (12, 1)
+    assert h(2, 3900, 3900, 2048) == (8, 1)
+    assert h(3, 4000, 4000, 512) == (8, 1)
+    assert h(2, 4096, 4096, 2048) == (12, 1)
+
+
 def test_transposed_a_takes_the_presplit_tile():
     """With A stored transposed (K >= 4096) the split modes take the pre-split
     tile, so the call needs workspace for the planes: (M + N) rows of 2 * K

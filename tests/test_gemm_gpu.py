@@ -346,6 +346,29 @@ def test_heuristic_presplit_for_underfilled_tiles(cc, dev, mode):
 
 
 @pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
+def test_heuristic_presplit_split_for_edge_tiles(cc, dev, mode):
+    """2900^2 x 3072 (144 partly filled 256x256 tiles) takes the pre-split
+    configuration with K split in three by default; bitwise the on-the-fly
+    256x256 tile at the same split (both cut K into 32-aligned slices of the
+    same length and sum the slices in the same order), and within the split
+    modes' componentwise bound of the FP64 product."""
+    planars, fly = PRESPLIT[mode]
+    M, N, K = 2900, 2900, 3072
+    cfg, split = cc.heuristic(cc._lib.MODES[mode], M, N, K)
+    assert cfg in planars and split == 3
+    rng = np.random.default_rng(405)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    for ta in (False, True):
+        Ad, Bd = _t(A.T.copy() if ta else A, dev), _t(B, dev)
+        got = cc.gemm(Ad, Bd, mode=mode, trans_a=ta).cpu().numpy()
+        want = cc.gemm(Ad, Bd, mode=mode, trans_a=ta, tile_cfg=fly[0], split_k=3).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), ta
+    _bound_check(got, exact, A, B, 2.0, 2.0**-24, extra=2.0**-15)
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "bf16x4"])
 def test_spread_load_tile_equals_ping_pong(cc, dev, mode):
     """The spread-load 256x256 tile (every wave copies first and issues its
     next loads among its MFMAs) computes the ping-pong tile's products in the
