@@ -272,17 +272,38 @@ hipError_t run_reduce_f64(const double* ws, double* C, int64_t ldc, int M, int N
   return launch_reduce<double>(ws, C, ldc, M, N, S, alpha, beta, act, s);
 }
 
-// Cache flush: overwrite a buffer larger than L2 + Infinity Cache.
+// Cache flush: overwrite a buffer larger than L2 + Infinity Cache, then read
+// it back.  The writes evict whatever the caches held; the reads evict the
+// writes' own dirty lines, so their write-back happens here and not inside
+// the timed call after the flush.  Overwriting alone left up to L2 + Infinity
+// Cache of dirty lines for the timed call to write back on its first
+// evictions: 57 us (12%) on the 256x256x1048576 BF16x3 kernel, which streams
+// 2 GiB (0.510 against 0.453 ms), 1% at 4096^3.
 __global__ void __launch_bounds__(256) flush_kernel(float4* __restrict__ buf, int64_t n, float v) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     buf[i] = make_float4(v, v, v, v);
 }
 
+// Reads the buffer flush_kernel filled with v > 0; the store is never taken
+// (a sum of positive values), it only keeps the loads.
+__global__ void __launch_bounds__(256) flush_read_kernel(float4* __restrict__ buf, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  float acc = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float4 w = buf[i];
+    acc += w.x + w.w;
+  }
+  if (acc < 0.f && (int64_t)threadIdx.x < n) buf[threadIdx.x] = make_float4(acc, acc, acc, acc);
+}
+
 hipError_t run_flush(void* buf, size_t bytes, hipStream_t s) {
   const int64_t n = (int64_t)(bytes / sizeof(float4));
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(flush_kernel, dim3(4096), dim3(256), 0, s, static_cast<float4*>(buf), n, 1.0f);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(flush_read_kernel, dim3(4096), dim3(256), 0, s, static_cast<float4*>(buf), n);
   return hipGetLastError();
 }
 
