@@ -157,30 +157,35 @@ constexpr int clamp_split(int split, int k, int min_depth, int max_split = 64) {
   if (split > max_split) split = max_split;
   return split < 1 ? 1 : split;
 }
-// An output side that is not a multiple of 256 (BF16x4: neither of 256 nor of
-// 128) leaves partly filled 256x256 tiles, which cost the on-the-fly tiles
-// more than the pre-split one, whose plane rows hold K rounded up to 32 and so
-// start on cache lines, where cfg 12 reads the FP32 operands in place, rows
-// straddling cache lines when the leading dimension (here M or N) is not a
-// multiple of 32.  BF16x4 outputs whose sides are multiples of 128 run
-// fastest on the 128x128 and 256x256 on-the-fly tiles (3200^2).
+// An output side that is not a multiple of 256 leaves partly filled 256x256
+// tiles, which cost the on-the-fly tiles more than the pre-split one, whose
+// plane rows hold K rounded up to 32 and so start on cache lines, where cfg 12
+// reads the FP32 operands in place, rows straddling cache lines when the
+// leading dimension (here M or N) is not a multiple of 32.
 constexpr bool tile_edges(int mode, int m, int n) {
-  const bool e256 = m % 256 != 0 || n % 256 != 0;
-  const bool e128 = m % 128 != 0 || n % 128 != 0;
-  return e256 && (mode != 3 || e128);
+  (void)mode;
+  return m % 256 != 0 || n % 256 != 0;
 }
 // BF16x3 / BF16x4 outputs whose 256x256 tiles do not give every CU a
-// workgroup.  Rules read off tuner tables of three shape sets on one MI355X
-// (cache flushed; every tile configuration at K splits 1, 2, 4, ...,
-// floor(CUs / tiles) and floor(2 CUs / tiles)): 94 shapes per mode with M = N
-// from 384 to 3072 and K from 512 to 32768 plus ten rectangular ones; 58 with
-// M = N from 448 to 3328 (not multiples of 256) and K from 768 to 24576 plus
-// ten rectangular ones; 45 with M = N from 320 to 3200 and K from 640 to
-// 20480 plus ten rectangular ones.  Against the fastest candidate of each
-// shape these picks are 1.3% / 1.9% / 2.3% slower in the geometric mean for
-// BF16x3 and 0.7% / 1.1% / 4.5% for BF16x4 (up to 1.25x on a few shapes);
-// the rules they replace were 6.6% / 7.8% / 9.4% and 10.0% / 13.6% / 14.7%
-// slower (up to 2.2x).
+// workgroup.  Rules read off full tuner tables on one MI355X (every tile
+// configuration at K splits 1, 2, 4, ..., floor(j CUs / tiles) for j = 1 to
+// 4; before each timed call 512 MiB overwritten and read back) of five shape
+// sets, 290 shapes per mode: M = N from 384 to 3072 x K from 512 to 32768; M
+// = N from 448 to 3328 (none a multiple of 256) x K from 768 to 24576; M = N
+// from 320 to 3200 x K from 640 to 20480 (each with ten rectangular shapes);
+// skinny products (M from 16 to 192 against N from 1024 to 16384); and M = N
+// from 2900 to 4000.  Against the fastest candidate these picks are 0.2-1.5%
+// slower per set in the geometric mean in BF16x3 and 0.3-2.9% in BF16x4,
+// where the rules read off tables measured under an overwrite-only flush were
+// 1.3-5.6% and 1.0-6.9%.
+//  - 256x256 tiles filling at least half the CUs and covering the output at
+//    least 88%, K >= 512: with edge tiles (tile_edges) the pre-split tile at
+//    the split below (2900^2 x 3072 to 8192: split 3, 0-2% from the fastest
+//    candidate against 7-13% for cfg 12; 3500^2 and 3700^2 BF16x4, K <=
+//    2048: 5-20% faster than 128x128), except BF16x4 with sides multiples of
+//    128 and K < 2048, which falls through to the 128x128 tiles; BF16x3 with
+//    sides multiples of 256 and K >= 1536 the spread-load tile at that split
+//    (3072^2 and 3328^2 x 1536 to 2048: 6-8% faster than pre-split).
 //  - 256x256 spread-load tile (cfg 12): K split for one round of workgroups
 //    (floor(CUs / tiles)) or for two (floor(2 CUs / tiles)), whichever makes
 //    rounds * ceil(K / split) + D * split smaller; D (400 BF16x3, 150 BF16x4)
@@ -196,33 +201,25 @@ constexpr bool tile_edges(int mode, int m, int n) {
 //    tile area, times the filled share of its rounds) is at least the 128x128
 //    tiles' (1152^2 to 1664^2: 81-86% against 95-99%, 128x128 tiles 3-26%
 //    faster; 1856^2: 82% against 82%, this tile 6-27% faster in BF16x3).
-//  - Before those two rules, 256x256 tiles filling at least half the CUs and
-//    covering the output at least 88%, K >= 512: with edge tiles
-//    (tile_edges) the pre-split tile at the split above (2900^2 x 3072 to
-//    8192: split 3, 0-2% from the fastest candidate against 7-13% for cfg
-//    12; 3500^2 and 3700^2 BF16x4, K <= 2048: 5-20% faster than 128x128);
-//    BF16x3 with sides multiples of 256 and K >= 1536 the spread-load tile
-//    at that split (3072^2 and 3328^2 x 1536 to 2048: 6-8% faster than
-//    pre-split).  A band table of M = N from 2900 to 3900 x K from 512 to
-//    8192 went from 1.8% / 4.5% slower than the fastest candidate in the
-//    geometric mean (BF16x3 / BF16x4) to 0.6% / 0.5%.
+//    The pre-split tile replaces it at the same split with edge tiles once
+//    M*N/(M+N) >= 768, and in BF16x4 with two rounds once it reaches 1400.
 //  - One round of unsplit 256x256 workgroups filling at least half the CUs,
 //    tiles covering the output at least 88%, K >= 512 (BF16x4: 1024): the
 //    pre-split tile (3500^2x3000 16-17% faster than cfg 12; 3072^2x1024
 //    BF16x3 14% faster than 128x128 tiles), except that BF16x4 with K <= 2048
 //    takes unsplit 128x128 tiles when they fill 60% of their slots
 //    (3328^2x1536: 7% faster).
-//  - 128x128 (cfg 11: 8 waves, two workgroups per CU): K split toward two
-//    workgroups per CU with slices at least 192 deep (from K = 1024; when
-//    the tiles alone give every CU a workgroup, from K = 2048), taken when
-//    that fills three quarters of the slots or every CU has a tile.  It ran
-//    0-30% faster than the 4-wave 128x128 tile (cfg 0) in BF16x4 and within
-//    3% of it in BF16x3.
-//  - Otherwise, or when the 128x128 tiles cover under 70% of the output,
-//    64x64 (cfg 1), K split toward three workgroups per CU with
-//    slices at least 128 deep while the tiles number fewer than twice the
-//    CUs (fewer than the CUs below K = 1024; 384^2x512: split 4, 1.7x faster
-//    than unsplit).
+//  - 128x128 tiles: K split toward two workgroups per CU with slices at least
+//    192 deep (from K = 1024; when the tiles alone give every CU a workgroup,
+//    from K = 2048), taken when that fills three quarters of the slots or
+//    every CU has a tile.  BF16x4 runs cfg 11 (8 waves, two workgroups per
+//    CU; 3-25% faster than the 4-wave cfg 0); BF16x3 cfg 0 when both sides
+//    are multiples of 128, else cfg 11.
+//  - Otherwise, or when the 128x128 tiles cover at most three quarters of the
+//    output, 64x64 (cfg 1), K split toward three workgroups per CU with
+//    slices at least 128 deep (a side below 128: four, 64 deep) while the
+//    tiles number fewer than twice the CUs (fewer than the CUs below K =
+//    1024; 384^2x512: split 4, 1.7x faster than unsplit).
 constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   const bool x4 = mode == 3;
   const int64_t lt = tiles_of(mode, 12, m, n);
@@ -241,7 +238,10 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   if (s12 == sb && s12 > 1 && !(k >= 2048 && k / s12 >= depth)) s12 = sa;
   const bool cover_ok = 25 * (int64_t)m * n >= 22 * lt * 65536;
   if (2 * lt >= cus && k >= 512 && cover_ok) {
-    if (tile_edges(mode, m, n)) return Choice{planar_cfg(mode), s12};
+    // (BF16x4 with sides multiples of 128 and K < 2048 falls through to the
+    // 128x128 tiles: 3200^2 x 512 to 1024 2-9% faster than pre-split.)
+    if (tile_edges(mode, m, n) && (!x4 || k >= 2048 || m % 128 != 0 || n % 128 != 0))
+      return Choice{planar_cfg(mode), s12};
     if (!x4 && m % 256 == 0 && n % 256 == 0 && k >= 1536) return Choice{12, s12};
   }
   const bool x4_c11 = x4 && 2 * t11 >= 3 * (int64_t)cus && k <= 2048 && 5 * w11 >= 3 * slots11;
@@ -251,12 +251,32 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
   // Useful work share, 256x256 against 128x128: M*N*s12 / (65536 * CUs *
   // rounds12) >= M*N*s11 / (16384 * slots11), with M*N cancelled.
   const bool share_ok = (int64_t)s12 * slots11 >= 4 * (int64_t)s11 * cus * cdiv(lt * s12, cus);
-  if (share_ok && 4 * lt * s12 >= 3 * (int64_t)cus && !x4_c11 && deep_ok) return Choice{12, s12};
-  // 128x128 tiles covering under 70% of the output (a side well below 128)
-  // give way to 64x64 tiles (192x16384x1024: 1.36x faster).
-  const bool c11_covers = 10 * (int64_t)m * n >= 7 * t11 * 16384;
-  if ((2 * w11 >= 3 * (int64_t)cus || t11 >= cus) && c11_covers) return Choice{11, s11};
+  if (share_ok && 4 * lt * s12 >= 3 * (int64_t)cus && !x4_c11 && deep_ok) {
+    // The pre-split tile at the same split: with edge tiles once M*N/(M+N)
+    // reaches 768 (1750^2 to 2600^2: 4-19% faster than cfg 12; 960^2, 1216^2
+    // and 500x1500 keep cfg 12, 3-31% faster), and in BF16x4 with two rounds
+    // of workgroups once it reaches 1400 (3072^2 and 3328^2 split 3: 1-6%).
+    const int64_t mn = (int64_t)m * n, m_n = (int64_t)m + n;
+    if (k >= 512 && ((tile_edges(mode, m, n) && mn >= 768 * m_n) ||
+                     (x4 && lt * s12 > cus && mn >= 1400 * m_n)))
+      return Choice{planar_cfg(mode), s12};
+    return Choice{12, s12};
+  }
+  // 128x128 tiles covering at most three quarters of the output (a side well
+  // below 128, or 192) give way to 64x64 tiles (192x16384x1024: 1.38x faster;
+  // 448^2, 77% covered, keeps 128x128 tiles).
+  const bool c11_covers = 4 * (int64_t)m * n > 3 * t11 * 16384;
+  // BF16x3 outputs whose sides are multiples of 128 take the 4-wave 128x128
+  // tile (cfg 0) at that split: 1-8% faster than cfg 11 on 53 such shapes;
+  // with edges cfg 11 is 4-21% faster (BF16x4: cfg 11 throughout).
+  if ((2 * w11 >= 3 * (int64_t)cus || t11 >= cus) && c11_covers)
+    return Choice{!x4 && m % 128 == 0 && n % 128 == 0 ? 0 : 11, s11};
   if (t1 >= 2 * cus || (t1 >= cus && k < 1024)) return Choice{1, 1};
+  // A side below 128 (at most two tile rows): toward four workgroups per CU,
+  // slices at least 64 deep (16 to 64 x 1024 to 16384 x 1024 to 16384: 1.6% /
+  // 1.8% slower than the fastest candidate in BF16x3 / BF16x4 against 5.5%
+  // aiming at three with 128-deep slices).
+  if (m < 128 || n < 128) return Choice{1, clamp_split(cdiv(4 * (int64_t)cus, t1), k, 64)};
   return Choice{1, clamp_split(cdiv(3 * (int64_t)cus, t1), k, 128)};
 }
 
@@ -396,14 +416,10 @@ constexpr Choice heuristic(int mode, int m, int n, int k, int ta, int cus) {
       const int64_t per_round = mode == 3 ? 6144 : 5504;
       if (k >= 512 && ((ta && k >= 4096) || (int64_t)cdiv(lt, cus) * per_round >= (int64_t)m + n))
         return Choice{planar_cfg(mode), 1};
-      // Edge tiles (tile_edges): pre-split whenever K >= 512, except BF16x3
-      // with one round of tiles and K < 2048 (3900x7900, 2000x16000 and
-      // 1500x20000, two rounds, 0-2% from the fastest candidate against
-      // 3-17% on the fly; 3900^2 and 4000^2, one round: BF16x3 2-5% slower
-      // pre-split below K = 2048, 0-10% faster from there; BF16x4 0-8% faster
-      // from K = 512).
-      if (k >= 512 && tile_edges(mode, m, n) && (mode == 3 || k >= 2048 || lt > cus))
-        return Choice{planar_cfg(mode), 1};
+      // Edge tiles (tile_edges): pre-split whenever K >= 512 (3900x7900,
+      // 2000x16000 and 1500x20000: 0-2% from the fastest candidate against
+      // 3-17% on the fly; 3900^2 and 4000^2 x 512 to 1024 BF16x3: 15% faster).
+      if (k >= 512 && tile_edges(mode, m, n)) return Choice{planar_cfg(mode), 1};
       return Choice{cfg, 1};
     }
     return split_medium(mode, m, n, k, cus);

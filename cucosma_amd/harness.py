@@ -424,17 +424,22 @@ def heuristic_shape_sets():
 
 def cfg_heuristic(cc, torch, args, emit):
     """The library's default (tile, split) against the autotuner's fastest
-    candidate (BF16x3 and BF16x4, or --modes): every tile configuration at K splits 1, 2,
-    4, ..., floor(CUs / tiles), floor(2 CUs / tiles) and the default's own,
-    cache flushed, the four fastest re-timed in alternating order (the
-    default's time is a single pass of --runs/20 calls unless it is among
-    them).  One line per shape (default time / fastest), then per set and
-    mode the geometric mean, the largest ratio and the count above 1.10."""
+    candidate (BF16x3 and BF16x4, or --modes): every tile configuration and
+    group size at K splits 1, 2, 4, ..., floor(j CUs / tiles) for j = 1 to 4
+    and the default's own, cache flushed.  When the default is not the
+    fastest, the two are timed again against each other, four rounds of
+    --runs/20 flushed calls each in alternating order, and the ratio is of
+    their medians: the tuner's minimum over many single passes is biased low
+    against one pass of the default.  One line per shape (default time /
+    fastest, and the tuner's own ratio), then per set and mode the geometric
+    mean, the largest ratio and the count above 1.10."""
     from . import autotune
 
     reps = max(3, args.runs // 20)
+    g = torch.Generator(device="cuda").manual_seed(7)
     for mode in (args.modes.split(",") if args.modes else ("bf16x3", "bf16x4")):
         mi = cc._lib.MODES[mode]
+        dt = torch.float64 if mode == "fp64" else torch.float32
         for name, shapes in heuristic_shape_sets().items():
             ratios = []
             for (M, N, K) in shapes:
@@ -445,11 +450,26 @@ def cfg_heuristic(cc, torch, args, emit):
                 mine = [c for c in ok if c["tile_cfg"] == cfg and c["split_k"] == split and c["group_m"] == 8]
                 if not mine:
                     continue
-                ratio = mine[0]["ms"] / best["ms"]
+                tuner_ratio = mine[0]["ms"] / best["ms"]
+                ms_mine, ms_best = mine[0]["ms"], best["ms"]
+                if mine[0] is not best:
+                    A = torch.rand((M, K), device="cuda", generator=g, dtype=dt) * 2 - 1
+                    B = torch.rand((K, N), device="cuda", generator=g, dtype=dt) * 2 - 1
+                    Cm = torch.empty((M, N), device="cuda", dtype=dt)
+                    t = {0: [], 1: []}
+                    picks = ((cfg, split, 0), (best["tile_cfg"], best["split_k"], best["group_m"]))
+                    for rnd in range(4):
+                        for j in ((0, 1) if rnd % 2 == 0 else (1, 0)):
+                            c, sk, gm = picks[j]
+                            t[j].extend(bench_gemm(cc, mode, A, B, Cm, 1.0, 0.0, reps, 1, True, tile_cfg=c,
+                                                   split_k=sk, group_m=gm).tolist())
+                    ms_mine, ms_best = float(np.median(t[0])), float(np.median(t[1]))
+                    del A, B, Cm
+                ratio = ms_mine / ms_best
                 ratios.append(ratio)
                 emit(dict(config="heuristic", set=name, mode=mode, M=M, N=N, K=K, tile_cfg=cfg, split_k=split,
-                          ms=mine[0]["ms"], best_tile_cfg=best["tile_cfg"], best_split_k=best["split_k"],
-                          best_group_m=best["group_m"], best_ms=best["ms"], ratio=ratio))
+                          ms=ms_mine, best_tile_cfg=best["tile_cfg"], best_split_k=best["split_k"],
+                          best_group_m=best["group_m"], best_ms=ms_best, ratio=ratio, tuner_ratio=tuner_ratio))
             if ratios:
                 emit(dict(config="heuristic_summary", set=name, mode=mode, shapes=len(ratios),
                           geomean=float(np.exp(np.mean(np.log(ratios)))), max=float(max(ratios)),

@@ -99,9 +99,10 @@ def test_heuristic_round_rules():
         assert h(mode, 5632, 5632, 2048) == ((12, 1) if mode == 2 else (8, 1))
         assert h(mode, 4096, 8192, 4096) == ((12, 1) if mode == 2 else (8, 1))
         # One partly filled round (144 tiles): the pre-split tile from K = 512
-        # (BF16x4: 1024) on, 128x128 tiles below.
+        # (BF16x4: 1024) on, 128x128 tiles below (BF16x3, sides multiples of
+        # 128: the 4-wave 128x128 tile).
         assert h(mode, 3072, 3072, 1024) == (8, 1)
-        assert h(mode, 3072, 3072, 256) == (11, 1)
+        assert h(mode, 3072, 3072, 256) == ((0, 1) if mode == 2 else (11, 1))
         assert h(mode, 8192, 8192, 8192) == (8, 1)
 
 
@@ -144,22 +145,42 @@ def test_heuristic_split_medium_rules():
     # ... or for two rounds when one leaves many CUs idle and K is deep.
     assert h(2, 2560, 2560, 8192) == (12, 2)
     assert h(2, 2560, 2560, 32768) == (12, 5)
-    assert h(3, 3072, 3072, 4096) == (12, 3)
+    # ... over pre-split operands with edge tiles once M*N/(M+N) >= 768, and
+    # in BF16x4 with two rounds once it reaches 1400; small edge outputs keep
+    # the on-the-fly tile.
+    assert h(2, 2432, 2432, 5120) == (8, 2)
+    assert h(2, 1856, 1856, 10240) == (8, 4)
+    assert h(3, 1856, 1856, 10240) == (8, 4)
+    assert h(2, 2176, 2176, 6144) == (8, 3)
+    assert h(3, 3072, 3072, 4096) == (8, 3)
+    assert h(3, 3328, 3328, 8192) == (8, 3)
+    assert h(2, 960, 960, 10240) == (12, 16)
+    assert h(2, 1216, 1216, 10240) == (12, 10)
     # BF16x4 takes 128x128 tiles where they are as fast and fill better.
     assert h(3, 2048, 2048, 2048) == (11, 2)
     assert h(3, 2560, 2560, 2048) == (11, 1)
     assert h(3, 1024, 1024, 8192) == (11, 8)
     assert h(3, 3328, 3328, 1536) == (11, 1)
     # 256x256 tiles covering under 88% of the output: 128x128 tiles, K split
-    # toward two workgroups per CU.
-    assert h(2, 384, 384, 16384) == (11, 56)
+    # toward two workgroups per CU (BF16x3 with sides multiples of 128: the
+    # 4-wave 128x128 tile at that split).
+    assert h(2, 384, 384, 16384) == (0, 56)
+    assert h(2, 1152, 1152, 3072) == (0, 6)
+    assert h(2, 2560, 2560, 1024) == (0, 1)
     assert h(3, 640, 640, 16384) == (11, 20)
     assert h(3, 1408, 1408, 6144) == (11, 4)
     # One unsplit round filling half the CUs or more: the pre-split tile.
     assert h(2, 3500, 3500, 3000) == (8, 1)
     assert h(3, 2900, 2900, 1100) == (8, 1)
-    # 128x128 tiles covering under 70% of the output: 64x64 tiles.
-    assert h(2, 4096, 32, 16384) == (1, 12)
+    # 128x128 tiles covering at most three quarters of the output: 64x64
+    # tiles; a side below 128: K split toward four workgroups per CU, slices
+    # at least 64 deep.
+    assert h(2, 4096, 32, 16384) == (1, 16)
+    assert h(2, 192, 16384, 1024) == (1, 1)
+    assert h(3, 192, 16384, 1024) == (1, 1)
+    assert h(2, 16, 16384, 16384) == (1, 4)
+    assert h(3, 16, 16384, 16384) == (1, 4)
+    assert h(2, 32, 1024, 1024) == (1, 16)
     # 64x64: K split toward three workgroups per CU, slices at least 128 deep.
     assert h(2, 384, 384, 512) == (1, 4)
     assert h(3, 1024, 1024, 1024) == (1, 3)
@@ -167,10 +188,11 @@ def test_heuristic_split_medium_rules():
 
 def test_heuristic_edge_tiles_presplit():
     """BF16 outputs with partly filled 256x256 tiles (a side not a multiple of
-    256; BF16x4: nor of 128) take the pre-split tile once those tiles fill half
-    the CUs; aligned outputs keep the on-the-fly tiles.  Every pick here was
-    the tuner's fastest configuration for the shape (cache flushed, one
-    MI355X; host code, nominal 256 CUs)."""
+    256) take the pre-split tile once those tiles fill half the CUs, except
+    BF16x4 outputs whose sides are multiples of 128 below K = 2048; aligned
+    outputs keep the on-the-fly tiles.  Every pick here was the tuner's
+    fastest configuration for the shape (cache flushed, one MI355X; host
+    code, nominal 256 CUs)."""
     import cucosma_amd
 
     h = cucosma_amd.heuristic
@@ -178,15 +200,16 @@ def test_heuristic_edge_tiles_presplit():
     assert h(2, 2900, 2900, 4096) == (8, 3)
     assert h(3, 2900, 2900, 4096) == (8, 3)
     assert h(3, 3700, 3700, 1536) == (8, 1)
-    # Sides multiples of 128: BF16x4 keeps 128x128 tiles.
+    # Sides multiples of 128: BF16x4 keeps 128x128 tiles below K = 2048.
     assert h(3, 3200, 3200, 1536) == (11, 1)
+    assert h(3, 3200, 3200, 4096) == (8, 3)
     # Aligned, BF16x3 from K = 1536: the spread-load tile.
     assert h(2, 3328, 3328, 1536) == (12, 1)
     assert h(2, 3072, 3072, 4096) == (12, 3)
     assert h(2, 3072, 3072, 1024) == (8, 1)
-    # Full rounds: pre-split with edges (BF16x3 with one round from K = 2048).
+    # Full rounds: pre-split with edges from K = 512.
     assert h(2, 3900, 7900, 1024) == (8, 1)
-    assert h(2, 3900, 3900, 1024) == (12, 1)
+    assert h(2, 3900, 3900, 1024) == (8, 1)
     assert h(2, 3900, 3900, 2048) == (8, 1)
     assert h(3, 4000, 4000, 512) == (8, 1)
     assert h(2, 4096, 4096, 2048) == (12, 1)
