@@ -786,8 +786,11 @@ int run_impl(const cosma_gemm_desc* d, bool use_scratch) {
   p.stream = s;
   size_t planes_off = 0;
   size_t need = workspace_need(d, cfg, split, &planes_off);
-  if (use_scratch && need > kScratchBytes && is_planar(d->mode, cfg) && d->tile_cfg < 0 && !d->a_planes &&
-      !d->b_planes) {
+  // A default pre-split choice whose planes do not fit the memory the call
+  // may use (the static array, or the caller's workspace) runs the on-the-fly
+  // tile at the same split instead.
+  const size_t room = use_scratch ? kScratchBytes : (d->workspace ? d->workspace_bytes : SIZE_MAX);
+  if (need > room && is_planar(d->mode, cfg) && d->tile_cfg < 0 && !d->a_planes && !d->b_planes) {
     cfg = large_cfg(d->mode);  // on-the-fly 256x256 tile: bitwise the pre-split result
     p.cfg = cfg;
     need = workspace_need(d, cfg, split, &planes_off);

@@ -124,7 +124,10 @@ int cosma_gemm_ex_static(const cosma_gemm_desc *d);
 size_t cosma_static_scratch_bytes(void);
 
 /* Bytes of workspace the call *d needs: split-K partials, then the planes of
- * operands split inside the call (0 when neither applies). */
+ * operands split inside the call (0 when neither applies).  When the library
+ * chose the pre-split tile (tile_cfg < 0) and d->workspace is smaller than
+ * this, the call runs the on-the-fly 256x256 tile at the same split instead,
+ * which needs only the partials and gives the same bits. */
 int cosma_workspace_query(const cosma_gemm_desc *d, size_t *bytes);
 
 /* Install a caller-owned device buffer used for split-K partials (and the

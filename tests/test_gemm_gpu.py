@@ -594,3 +594,45 @@ def test_fp64_device_reference_matches_host_oracle(cc, dev):
     ref = oracle.gemm_f64(A, B)
     absprod = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
     assert np.all(np.abs(R - ref) <= 4 * 3000 * 2.0**-53 * absprod)
+
+
+def test_small_caller_workspace_runs_on_the_fly_tile(cc, dev):
+    """A default pre-split choice whose planes do not fit the caller's
+    workspace runs the on-the-fly 256x256 tile at the same split: the call
+    succeeds and is bitwise the library-workspace call.  A workspace too small
+    for the split-K partials as well is refused before any launch."""
+    from cucosma_amd._lib import MODES, GemmDesc
+
+    def desc(A, B, Cm, ws=None, ws_bytes=0):
+        d = GemmDesc()
+        cc.lib.cosma_gemm_desc_init(C.byref(d))
+        d.mode = MODES["bf16x3"]
+        d.m, d.k = A.shape
+        d.n = B.shape[1]
+        d.a, d.lda = A.data_ptr(), A.shape[1]
+        d.b, d.ldb = B.data_ptr(), B.shape[1]
+        d.c, d.ldc = Cm.data_ptr(), Cm.shape[1]
+        d.alpha, d.beta = 1.0, 0.0
+        if ws is not None:
+            d.workspace, d.workspace_bytes = ws.data_ptr(), ws_bytes
+        d.stream = torch.cuda.current_stream().cuda_stream
+        return d
+
+    g = torch.Generator(device="cpu").manual_seed(406)
+    for (M, N, K, split) in ((3072, 3072, 1024, 1), (2900, 2900, 3072, 3)):
+        assert cc.heuristic(cc._lib.MODES["bf16x3"], M, N, K) == (8, split)
+        A = (torch.rand(M, K, generator=g) * 2 - 1).to(dev)
+        B = (torch.rand(K, N, generator=g) * 2 - 1).to(dev)
+        ref = torch.empty(M, N, device=dev)
+        assert cc.lib.cosma_gemm_ex(C.byref(desc(A, B, ref))) == 0
+        partials = (split * M * N * 4) if split > 1 else 0
+        ws = torch.empty(max(partials, 1 << 20), dtype=torch.uint8, device=dev)
+        got = torch.empty(M, N, device=dev)
+        assert cc.lib.cosma_gemm_ex(C.byref(desc(A, B, got, ws, ws.numel()))) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(ref, got), (M, N, K)
+        if split > 1:
+            Cm = torch.full((M, N), 7.0, device=dev)
+            assert cc.lib.cosma_gemm_ex(C.byref(desc(A, B, Cm, ws, partials - 256))) == -1
+            torch.cuda.synchronize()
+            assert bool((Cm == 7.0).all())
