@@ -95,8 +95,7 @@ constexpr int tiles_of(int mode, int cfg, int m, int n) {
 //    4096x8192x4096 tie or lose and run on the fly; 5632^2x2048 lost 1% and
 //    is just outside).  5504 >= 2^24 / 3072, so every output with
 //    M*N/(M+N) >= 3072 also qualifies (BF16x4: 6144).  Outputs with edge
-//    tiles (tile_edges) pre-split from K = 512 (BF16x3 with one round of
-//    tiles: from K = 2048).  The 256x256 tile beat
+//    tiles (tile_edges) pre-split from K = 512.  The 256x256 tile beat
 //    every smaller tile from 2048^3 up.  Smaller outputs: split_medium;
 //  - FP32 / FP64 large outputs (at least two workgroups per CU): FP32 256x128
 //    in 16 waves of 64x32 with the next stage held in registers (four waves
