@@ -272,9 +272,9 @@ constexpr Choice split_medium(int mode, int m, int n, int k, int cus) {
     return Choice{!x4 && m % 128 == 0 && n % 128 == 0 ? 0 : 11, s11};
   if (t1 >= 2 * cus || (t1 >= cus && k < 1024)) return Choice{1, 1};
   // A side below 128 (at most two tile rows): toward four workgroups per CU,
-  // slices at least 64 deep (16 to 64 x 1024 to 16384 x 1024 to 16384: 1.6% /
-  // 1.8% slower than the fastest candidate in BF16x3 / BF16x4 against 5.5%
-  // aiming at three with 128-deep slices).
+  // slices at least 64 deep (the skinny set, M from 16 to 192: 2.1% / 1.8%
+  // slower than the fastest candidate in BF16x3 / BF16x4, against 5.3% /
+  // 5.5% aiming at three with 128-deep slices).
   if (m < 128 || n < 128) return Choice{1, clamp_split(cdiv(4 * (int64_t)cus, t1), k, 64)};
   return Choice{1, clamp_split(cdiv(3 * (int64_t)cus, t1), k, 128)};
 }
